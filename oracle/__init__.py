@@ -1,0 +1,149 @@
+"""FP64 CPU oracle of the path stage: TEST INFRASTRUCTURE ONLY.
+
+Only the test suite, the smoke check and the benchmark's CPU-baseline leg may import this package, and only as the
+checker (or the timed CPU baseline), never as the thing measured or shipped. The product (``feffgpu_amd``) never
+imports it.
+
+``liborc.so`` (built from ``feffref.c`` by ``make -C oracle``) is an FP64 restatement of feffgpu's literal FEFF
+port ``genfmt.PathStage`` (src/feffgpu/genfmt.py:279-461) with the leg geometry and fine-grid evaluation of
+``GPUPathStage`` (src/feffgpu/stage.py:147-206, 353-388). It parses phase.pad itself, so it shares no code with the
+product. What pins each function is listed in tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborc.so"
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def build(force: bool = False) -> Path:
+    """Compile liborc.so with gcc (make -C oracle)."""
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "feffref.c").stat().st_mtime:
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_info.argtypes = [ctypes.c_void_p, _ip]
+        L.orc_tables.argtypes = [ctypes.c_void_p, _ip, _dp, _dp]
+        L.orc_cchi.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _lp, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_chi_fine.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_chi_paths.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip,
+                                    _ip, _ip, _lp, _dp, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+class OraclePhase:
+    """A phase.pad file loaded by the oracle's own parser."""
+
+    def __init__(self, path):
+        self.path = str(path)
+        h = ctypes.c_void_p()
+        _check(lib().orc_load(self.path.encode(), ctypes.byref(h)))
+        self.h = h
+        info = np.zeros(8, np.int32)
+        lib().orc_info(self.h, _p(info, _ip))
+        self.ne, self.ne1, self.nph, self.ihole, self.ltot, self.lmaxp1, self.kinit, self.linit = (int(v) for v in info)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_free(self.h)
+            self.h = None
+
+    def tables(self):
+        """lmax (ne1, nph+1), xk (ne1,) 1/bohr, ck (ne1,) complex, as the oracle derives them."""
+        lm = np.zeros((self.ne1, self.nph + 1), np.int32)
+        xk = np.zeros(self.ne1)
+        ck = np.zeros(2 * self.ne1)
+        lib().orc_tables(self.h, _p(lm, _ip), _p(xk, _dp), _p(ck, _dp))
+        return lm, xk, ck[0::2] + 1j * ck[1::2]
+
+    def cchi(self, coords, ipots, iorder=2):
+        """One path: coords (nleg, 3) A, absorber first at the origin; ipots (nleg,). Returns (cchi (ne1,) complex,
+        reff A, geometry dict with ri (A), beta, eta per leg)."""
+        c = np.ascontiguousarray(coords, dtype=np.float64)
+        ip = np.ascontiguousarray(ipots, dtype=np.int64)
+        n = c.shape[0]
+        out = np.zeros(2 * self.ne1)
+        reff = np.zeros(1)
+        geom = np.zeros(3 * n)
+        _check(lib().orc_cchi(self.h, n, _p(c, _dp), _p(ip, _lp), int(iorder), _p(out, _dp), _p(reff, _dp), _p(geom, _dp)))
+        return out[0::2] + 1j * out[1::2], float(reff[0]), dict(ri=geom[:n], beta=geom[n:2 * n], eta=geom[2 * n:])
+
+    def chi_fine(self, cchi, reff, q, sigma2=None):
+        cc = np.ascontiguousarray(np.stack([np.asarray(cchi).real, np.asarray(cchi).imag], -1), dtype=np.float64)
+        cc = cc.reshape(-1, self.ne1, 2)
+        B = cc.shape[0]
+        rf = np.ascontiguousarray(np.broadcast_to(np.asarray(reff, np.float64), (B,)))
+        s2 = None if sigma2 is None else np.ascontiguousarray(np.broadcast_to(np.asarray(sigma2, np.float64), (B,)))
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros((B, len(qq)))
+        _check(lib().orc_chi_fine(self.h, B, _p(cc, _dp), _p(rf, _dp), _p(s2, _dp), len(qq), _p(qq, _dp), _p(out, _dp)))
+        return out
+
+
+def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, nseg=1, edge=None, sigma2=None,
+              cell=None, iorder=2, threads=None, want_paths=False):
+    """The fused path-list evaluation in FP64 (see orc_chi_paths in feffref.c). Returns (chi_seg (nseg, NK),
+    chi_path (P, NK) or None, reff (P,))."""
+    if not isinstance(phases, (list, tuple)):
+        phases = [phases]
+    P = int(len(nleg))
+    pos = np.ascontiguousarray(pos, np.float64)
+    atoms = np.ascontiguousarray(atoms, np.int32)
+    nmax = atoms.shape[1] + 1
+    nleg = np.ascontiguousarray(nleg, np.int32)
+    absorber = np.ascontiguousarray(absorber, np.int32)
+    ipots = np.ascontiguousarray(ipots, np.int64)
+    deg = np.ascontiguousarray(deg, np.float64)
+    segment = np.zeros(P, np.int32) if segment is None else np.ascontiguousarray(segment, np.int32)
+    edge = None if edge is None else np.ascontiguousarray(edge, np.int32)
+    sigma2 = None if sigma2 is None else np.ascontiguousarray(np.broadcast_to(np.asarray(sigma2, np.float64), (P,)))
+    q = np.ascontiguousarray(q, np.float64)
+    if cell is None:
+        cellv, ncell = None, 0
+    else:
+        cellv = np.ascontiguousarray(np.asarray(cell, np.float64).reshape(-1))
+        ncell = cellv.size
+        if ncell not in (1, 9):
+            raise ValueError("cell must be a cubic length or a 3 x 3 matrix")
+    handles = (ctypes.c_void_p * len(phases))(*[ph.h.value for ph in phases])
+    chi_seg = np.zeros((nseg, len(q)))
+    chi_path = np.zeros((P, len(q))) if want_paths else None
+    reff = np.zeros(P)
+    nthr = int(threads or os.cpu_count() or 1)
+    _check(lib().orc_chi_paths(handles, len(phases), P, nmax, _p(pos, _dp), _p(atoms, _ip), _p(nleg, _ip), _p(absorber, _ip),
+                               _p(ipots, _lp), _p(deg, _dp), _p(segment, _ip), _p(edge, _ip), _p(sigma2, _dp), int(nseg),
+                               len(q), _p(q, _dp), _p(cellv, _dp), ncell, int(iorder), nthr, _p(chi_seg, _dp),
+                               _p(chi_path, _dp), _p(reff, _dp)))
+    return chi_seg, chi_path, reff
