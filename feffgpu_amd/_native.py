@@ -1,0 +1,130 @@
+"""ctypes binding of the HIP library libfeffgpu_amd.so (the C ABI of include/feffgpu_amd.h).
+
+The library is built in-tree by ``feffgpu_amd.build`` (hipcc, gfx950). Loading fails loudly when it is missing:
+there is no CPU fallback of the path stage in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import tables as _tables
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libfeffgpu_amd.so"
+
+FM_MAX_LEGS, FM_MAX_CHAN, FM_MAX_POT, FM_NREP = 9, 4, 16, 20
+FM_BUCKET_K_MS, FM_BUCKET_LM_MS, FM_BUCKET_K_SS = 0, 1, 2
+FM_OK, FM_EINVAL, FM_EUNSUPPORTED, FM_EHIP = 0, -1, -2, -3
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+
+
+class FmEdge(ctypes.Structure):
+    _fields_ = [("E", _i32), ("L1", _i32), ("nph1", _i32), ("ilinit", _i32), ("nchan", _i32),
+                ("chan_il", _i32 * FM_MAX_CHAN), ("lmax1", _i32 * FM_MAX_POT),
+                ("lmax", _vp), ("ck", _vp), ("ckmxk", _vp), ("kf", _vp), ("tl32", _vp), ("tl64", _vp),
+                ("chw32", _vp), ("chw64", _vp), ("rotc", _vp), ("rotn", _vp)]
+
+
+class FmPaths(ctypes.Structure):
+    _fields_ = [("npath", _i32), ("nmax", _i32), ("pos", _vp), ("atoms", _vp), ("nleg", _vp), ("absorber", _vp),
+                ("ipots", _vp), ("deg", _vp), ("edge", _vp), ("sigma2", _vp), ("cell_kind", _i32),
+                ("cell", ctypes.c_double * 9), ("cellinv", ctypes.c_double * 9)]
+
+
+class FmGrid(ctypes.Structure):
+    _fields_ = [("nk", _i32), ("q", _vp), ("jq", _vp), ("fq", _vp)]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NativeError(f"{LIB_PATH} is missing: build it with `python -m feffgpu_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.fm_last_error.restype = ctypes.c_char_p
+        L.fm_version.restype = ctypes.c_char_p
+        L.fm_cchi.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp]
+        L.fm_cchi_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fm_chi_fine.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _vp, _vp, ctypes.POINTER(FmGrid), _vp, _vp]
+        L.fm_chi_fine_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _vp, _vp, ctypes.POINTER(FmGrid), _vp,
+                                           _vp, _vp, _vp]
+        L.fm_chi_paths.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid), _i32,
+                                   _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                   _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fm_chi_paths_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths),
+                                            ctypes.POINTER(FmGrid), _i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
+                                            _vp, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != FM_OK:
+        msg = lib().fm_last_error().decode()
+        if rc == FM_EUNSUPPORTED:
+            raise NotImplementedError(f"{what}: {msg}")
+        if rc == FM_EINVAL:
+            raise ValueError(f"{what}: {msg}")
+        raise NativeError(f"{what}: {msg}")
+
+
+def ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class DeviceEdge:
+    """The tables of one edge uploaded to a device, with the fm_edge struct that points at them."""
+
+    def __init__(self, T: _tables.EdgeTables, device):
+        dev = torch.device(device)
+        self.host = T
+
+        def put(a, dtype):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev).contiguous()
+
+        def cplx(a):
+            a = np.asarray(a)
+            return np.stack([a.real, a.imag], -1)
+
+        self.lmax = put(T.lmax, torch.int32)
+        self.ck = put(cplx(T.ck), torch.float64)
+        self.ckmxk = put(cplx(T.ckmxk), torch.float64)
+        self.kf = put(T.kf, torch.float64)
+        tl = cplx(np.transpose(T.tl, (0, 1, 2)))          # (nph1, L1+1, E, 2)
+        self.tl64 = put(tl, torch.float64)
+        self.tl32 = put(tl, torch.float32)
+        chw = cplx(T.chw)
+        self.chw64 = put(chw, torch.float64)
+        self.chw32 = put(chw, torch.float32)
+        self.rotc = put(T.rotc, torch.float64)
+        self.rotn = put(T.rotn, torch.float64)
+        s = FmEdge()
+        s.E, s.L1, s.nph1, s.ilinit, s.nchan = T.E, T.L1, T.nph1, T.ilinit, len(T.chan_il)
+        if T.nph1 > FM_MAX_POT:
+            raise NotImplementedError(f"more than {FM_MAX_POT} potentials")
+        for i, v in enumerate(T.chan_il):
+            s.chan_il[i] = v
+        for i, v in enumerate(T.lmax1):
+            s.lmax1[i] = int(v)
+        s.lmax, s.ck, s.ckmxk, s.kf = (t.data_ptr() for t in (self.lmax, self.ck, self.ckmxk, self.kf))
+        s.tl32, s.tl64, s.chw32, s.chw64 = (t.data_ptr() for t in (self.tl32, self.tl64, self.chw32, self.chw64))
+        s.rotc, s.rotn = self.rotc.data_ptr(), self.rotn.data_ptr()
+        self.struct = s

@@ -1,0 +1,220 @@
+"""The fused path stage over a path list: positions in, chi per segment and per path out, differentiable in the
+positions.
+
+This is the path-list form of feffgpu's forward model (``PathSpectra.spectra`` and
+``MultipleScattering.signals/path_signals``, src/feffgpu/exafs/forward.py:143-163, 657-685, and the sum of
+``chain.run``, src/feffgpu/chain.py:45-49): every path p is a sequence of atom indices after its absorber; its
+coordinates are ``[0, wrap(pos[atoms[p]] - pos[absorber[p]])]`` (minimum image, src/feffgpu/cell.py:26-35); its
+chi on the output grid q is the path stage's chi_fine (src/feffgpu/stage.py:372-388) times an optional Debye-Waller
+factor exp(-2 sigma^2 q^2); and ``chi_seg[s] = sum_{p in s} deg[p] chi_p`` (index_add in the reference; here a
+deterministic ordered sum).
+
+A :class:`PathPlan` holds a path list on the device together with its bucketing (paths grouped by the kernel
+variant their edge and number of legs need, longest first), the segment lists and the tables of every edge; its
+``forward(pos)`` is one call into the C ABI. ``chi_paths`` is the one-shot functional form.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .pad import load_phase
+from .tables import edge_tables, energies_needed, interpolation
+
+
+def _cell_struct(cell):
+    """(kind, cell[9], inverse[9]) of a cluster (None), a cubic box (a number) or a 3 x 3 cell (rows = vectors)."""
+    if cell is None:
+        return 0, np.zeros(9), np.zeros(9)
+    c = np.asarray(cell.detach().cpu() if torch.is_tensor(cell) else cell, float)
+    if c.ndim == 0 or c.size == 1:
+        v = np.zeros(9)
+        v[0] = float(c.reshape(-1)[0])
+        return 1, v, np.zeros(9)
+    if c.shape == (3,):
+        c = np.diag(c)
+    if c.shape != (3, 3):
+        raise ValueError("cell must be None, a cubic length, 3 lengths or a 3 x 3 matrix")
+    return 2, c.reshape(-1).copy(), np.linalg.inv(c).reshape(-1).copy()
+
+
+@dataclass
+class Buckets:
+    order: np.ndarray          # permutation of the paths
+    offset: np.ndarray         # (nb + 1,)
+    kind: np.ndarray           # (nb,)
+    nlegmax: np.ndarray        # (nb,)
+
+
+def bucket_paths(nleg: np.ndarray, edge: np.ndarray, ilinit: list) -> Buckets:
+    """Group the paths by kernel variant: (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (K/L1 edge, 2 legs); inside
+    a bucket the paths are ordered by decreasing number of legs (longest first, so the tail of a launch is short)."""
+    nleg = np.asarray(nleg)
+    edge = np.asarray(edge)
+    il = np.asarray(ilinit)[edge]
+    kind = np.where(nleg == 2, np.where(il <= 1, N.FM_BUCKET_K_SS, -1),
+                    np.where(il <= 1, N.FM_BUCKET_K_MS, N.FM_BUCKET_LM_MS))
+    if (kind < 0).any():
+        raise NotImplementedError("two-leg paths of L and M edges are not implemented on the GPU")
+    order = np.lexsort((np.arange(len(nleg)), -nleg, kind)).astype(np.int32)
+    ks = kind[order]
+    kinds = [k for k in (N.FM_BUCKET_K_MS, N.FM_BUCKET_LM_MS, N.FM_BUCKET_K_SS) if (ks == k).any()]
+    off = [0]
+    nlm = []
+    for k in kinds:
+        sel = ks == k
+        off.append(off[-1] + int(sel.sum()))
+        nlm.append(int(nleg[order][sel].max()))
+    return Buckets(order, np.array(off, np.int32), np.array(kinds, np.int32), np.array(nlm, np.int32))
+
+
+class PathPlan:
+    """A path list resident on the GPU, ready for repeated evaluation at new positions.
+
+    phases: a phase file (path or PhaseData) or a list of them (one per edge; ``edge[p]`` picks one).
+    q: the output grid (1/A). atoms (P, nmax-1) scatterer indices (padding beyond nleg-1 is ignored), nleg (P,),
+    absorber (P,), ipots (P, nmax) (absorber first), deg (P,), segment (P,) (default 0), edge (P,) (default 0),
+    sigma2 (P,) or a number (default none). cell: None, a cubic length or a 3 x 3 cell. precision 32 or 64.
+    """
+
+    def __init__(self, phases, q, atoms, nleg, absorber, ipots, deg, segment=None, nseg=None, edge=None, sigma2=None,
+                 cell=None, precision=32, device="cuda", iorder=2):
+        self.device = torch.device(device)
+        if self.device.type != "cuda" or not torch.cuda.is_available():
+            raise RuntimeError("PathPlan runs on a ROCm GPU only")
+        if precision not in (32, 64):
+            raise ValueError("precision must be 32 or 64")
+        self.precision, self.iorder = int(precision), int(iorder)
+        self.rdtype = torch.float32 if precision == 32 else torch.float64
+        phases = phases if isinstance(phases, (list, tuple)) else [phases]
+        self.phases = [load_phase(p) for p in phases]
+        if not 1 <= len(self.phases) <= 4:
+            raise ValueError("one to four edges per plan")
+        self.q = np.ascontiguousarray(np.asarray(q, float))
+        nk = len(self.q)
+        atoms = np.asarray(atoms, np.int64)
+        nleg = np.asarray(nleg, np.int64)
+        Pn = len(nleg)
+        if atoms.ndim != 2 or atoms.shape[0] != Pn:
+            raise ValueError("atoms must be (P, nmax-1)")
+        nmax = atoms.shape[1] + 1
+        if not 2 <= nmax <= N.FM_MAX_LEGS:
+            raise ValueError("nmax out of range 2..9")
+        if Pn and (nleg.min() < 2 or nleg.max() > nmax):
+            raise ValueError(f"nleg must be within 2..{nmax}")
+        ipots = np.asarray(ipots, np.int64)
+        if ipots.shape != (Pn, nmax):
+            raise ValueError("ipots must be (P, nmax)")
+        edge = np.zeros(Pn, np.int64) if edge is None else np.asarray(edge, np.int64)
+        if Pn and (edge.min() < 0 or edge.max() >= len(self.phases)):
+            raise ValueError("edge index out of range")
+        segment = np.zeros(Pn, np.int64) if segment is None else np.asarray(segment, np.int64)
+        self.nseg = int(nseg if nseg is not None else (segment.max() + 1 if Pn else 1))
+        if Pn and (segment.min() < 0 or segment.max() >= self.nseg):
+            raise ValueError("segment index out of range")
+        deg = np.asarray(deg, np.float64)
+        for e, ph in enumerate(self.phases):
+            sel = edge == e
+            if sel.any():
+                used = np.concatenate([ipots[sel][np.arange(nmax)[None, :] < nleg[sel][:, None]]])
+                if used.min() < 0 or used.max() > ph.nph:
+                    raise ValueError(f"potential index out of range 0..{ph.nph} for edge {e}")
+        # per-edge tables on the energies the grid reads
+        self.tables, self.edges = [], []
+        jq, fq = [], []
+        for ph in self.phases:
+            E = energies_needed(ph, self.q)
+            T = edge_tables(ph, E)
+            self.tables.append(T)
+            self.edges.append(N.DeviceEdge(T, self.device))
+            j, f = interpolation(T.kf_full, self.q)
+            jq.append(j)
+            fq.append(f)
+        self.edge_array = (N.FmEdge * len(self.edges))(*[e.struct for e in self.edges])
+        dev = self.device
+        self.q_t = torch.tensor(self.q, device=dev)
+        self.jq_t = torch.tensor(np.stack(jq), device=dev, dtype=torch.int32)
+        self.fq_t = torch.tensor(np.stack(fq), device=dev, dtype=torch.float64)
+        self.grid = N.FmGrid()
+        self.grid.nk, self.grid.q, self.grid.jq, self.grid.fq = nk, self.q_t.data_ptr(), self.jq_t.data_ptr(), self.fq_t.data_ptr()
+        # path arrays
+        self.npath, self.nmax, self.nk = Pn, nmax, nk
+        self.nleg_h, self.edge_h, self.segment_h, self.deg_h = nleg, edge, segment, deg
+        absorber = np.asarray(absorber, np.int64)
+        if absorber.shape != (Pn,):
+            raise ValueError("absorber must be (P,)")
+        used_atoms = atoms[np.arange(nmax - 1)[None, :] < (nleg[:, None] - 1)] if Pn else np.zeros(0, np.int64)
+        self.max_atom = int(max(used_atoms.max() if used_atoms.size else -1, absorber.max() if Pn else -1))
+        if Pn and min(int(absorber.min()), int(used_atoms.min()) if used_atoms.size else 0) < 0:
+            raise ValueError("negative atom index")
+        atoms = np.where(np.arange(nmax - 1)[None, :] < (nleg[:, None] - 1), atoms, 0)
+        i32 = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev, dtype=torch.int32)
+        self.atoms_t = i32(atoms if atoms.size else np.zeros((0, nmax - 1)))
+        self.nleg_t = i32(nleg)
+        self.absorber_t = i32(np.asarray(absorber, np.int64))
+        self.ipots_t = i32(ipots)
+        self.deg_t = torch.tensor(deg, device=dev, dtype=torch.float64)
+        self.edge_t = i32(edge)
+        if sigma2 is None:
+            self.sigma2_t = None
+        else:
+            s2 = np.broadcast_to(np.asarray(sigma2, float), (Pn,))
+            self.sigma2_t = torch.tensor(np.ascontiguousarray(s2), device=dev, dtype=torch.float64)
+        self.cell_kind, self.cell, self.cellinv = _cell_struct(cell)
+        self.buckets = bucket_paths(nleg, edge, [T.ilinit for T in self.tables])
+        self.order_t = i32(self.buckets.order)
+        seg_order = np.argsort(segment, kind="stable")
+        seg_ptr = np.zeros(self.nseg + 1, np.int64)
+        np.add.at(seg_ptr, segment + 1, 1)
+        self.seg_ptr_t = i32(np.cumsum(seg_ptr))
+        self.seg_paths_t = i32(seg_order)
+        self.chi_path = torch.empty(Pn, nk, dtype=self.rdtype, device=dev)
+        self.reff = torch.empty(Pn, dtype=torch.float64, device=dev)
+        self._h_off = (N._i32 * len(self.buckets.offset))(*self.buckets.offset.tolist())
+        self._h_kind = (N._i32 * max(1, len(self.buckets.kind)))(*self.buckets.kind.tolist())
+        self._h_nlm = (N._i32 * max(1, len(self.buckets.nlegmax)))(*self.buckets.nlegmax.tolist())
+
+    def _paths_struct(self, pos64):
+        s = N.FmPaths()
+        s.npath, s.nmax = self.npath, self.nmax
+        s.pos = pos64.data_ptr()
+        s.atoms, s.nleg, s.absorber, s.ipots = (t.data_ptr() for t in (self.atoms_t, self.nleg_t, self.absorber_t, self.ipots_t))
+        s.deg, s.edge = self.deg_t.data_ptr(), self.edge_t.data_ptr()
+        s.sigma2 = N.ptr(self.sigma2_t)
+        s.cell_kind = self.cell_kind
+        for i in range(9):
+            s.cell[i] = float(self.cell[i])
+            s.cellinv[i] = float(self.cellinv[i])
+        return s
+
+    def forward(self, pos, want_seg=True):
+        """pos (N, 3) A. Returns (chi_seg (nseg, NK) float64, chi_path (P, NK) in the plan's precision, reff (P,) A).
+        chi_path and reff are the plan's own buffers, overwritten by the next call."""
+        pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
+        if pos64.dim() != 2 or pos64.shape[1] != 3:
+            raise ValueError("pos must be (N, 3)")
+        if self.max_atom >= pos64.shape[0]:
+            raise ValueError(f"the path list refers to atom {self.max_atom} but pos has {pos64.shape[0]} rows")
+        chi_seg = torch.empty(self.nseg, self.nk, dtype=torch.float64, device=self.device) if want_seg else None
+        if self.npath:
+            ps = self._paths_struct(pos64)
+            N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
+                                         self.order_t.data_ptr(), len(self.buckets.kind), self._h_off, self._h_kind,
+                                         self._h_nlm, self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(),
+                                         N.ptr(chi_seg), self.chi_path.data_ptr(), self.reff.data_ptr(),
+                                         N.stream_of(self.device)), "fm_chi_paths")
+        elif want_seg:
+            chi_seg.zero_()
+        return chi_seg, self.chi_path, self.reff
+
+
+def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, nseg=None, edge=None, sigma2=None,
+              cell=None, precision=32, device="cuda"):
+    """One-shot fused evaluation: returns (chi_seg (nseg, NK), chi_path (P, NK), reff (P,))."""
+    plan = PathPlan(phases, q, atoms, nleg, absorber, ipots, deg, segment=segment, nseg=nseg, edge=edge,
+                    sigma2=sigma2, cell=cell, precision=precision, device=device)
+    seg, path, reff = plan.forward(pos)
+    return seg, path.clone(), reff.clone()

@@ -1,0 +1,135 @@
+/*
+ * feffgpu_amd: the path stage of feffgpu (Rehr-Albers multiple-scattering path amplitudes and their sum to
+ * chi(k)) as hand-written HIP kernels for MI355X (gfx950), behind a plain C ABI.
+ *
+ * Every pointer in these structs and calls is a DEVICE pointer owned by the caller, unless marked "host". No call
+ * below allocates device memory: workspaces are passed in, so every call can be captured in a hipGraph. All calls
+ * are asynchronous on the given stream (a hipStream_t, NULL for the default stream) and return 0 on success or a
+ * negative code; fm_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Units as in FEFF and feffgpu: coordinates and positions in Angstrom, momenta in 1/bohr inside the tables, the
+ * output grid q in 1/Angstrom. Precision 32 = FP32 amplitudes with FP64 leg geometry and FP64 path phase reduced
+ * mod 2 pi before the FP32 sine; precision 64 = FP64 throughout.
+ */
+#ifndef FEFFGPU_AMD_H
+#define FEFFGPU_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_MAX_LEGS 9          /* FEFF's own limit (src/feffgpu/pathfinder.py:26) */
+#define FM_MAX_CHAN 4          /* distinct final-state l of the termination */
+#define FM_MAX_POT 16          /* potentials per phase file */
+#define FM_NREP 20             /* independent (lambda1, lambda2) pairs of a 6 x 6 scattering matrix */
+
+enum { FM_OK = 0, FM_EINVAL = -1, FM_EUNSUPPORTED = -2, FM_EHIP = -3 };
+
+/* Device tables of one absorber edge, built once from a phase.pad (GPUPathStage.__init__, src/feffgpu/stage.py:97-125,
+ * and the lambda / rotation bookkeeping of src/feffgpu/stage.py:41-93, 128-144). */
+typedef struct fm_edge {
+    int32_t E;                   /* phase-grid energies evaluated (first E of ne1) */
+    int32_t L1;                  /* largest il = lmax + 1 over those energies and potentials */
+    int32_t nph1;                /* potentials 0..nph */
+    int32_t ilinit;              /* initial-state l + 1 (1: K/L1, 2: L2/L3, 3: M4/M5) */
+    int32_t nchan;               /* distinct final-state channels */
+    int32_t chan_il[FM_MAX_CHAN];/* il = l + 1 of each channel */
+    int32_t lmax1[FM_MAX_POT];   /* host: max over the E energies of lmax + 1 per potential */
+    const int32_t *lmax;         /* [nph1][E] per-energy partial-wave limit */
+    const double *ck;            /* [E][2] complex momentum (1/bohr) */
+    const double *ckmxk;         /* [E][2] ck - xk */
+    const double *kf;            /* [E] xk / BOHR (1/A) */
+    const float *tl32;           /* [nph1][L1 + 1][E][2] j-averaged t-matrix (l + 1) t(-l) + l t(l), index il */
+    const double *tl64;
+    const float *chw32;          /* [nchan][E][2] termination weight sum_k bdiag_k rkk_k^2 of each channel */
+    const double *chw64;
+    const double *rotc;          /* [L1 + 1][49][3] d-recursion coefficients P, Q, R per l and (m1 + 3) * 7 + m2 + 3 */
+    const double *rotn;          /* [L1 + 1][16] normalisation xnlm(il, |m1|) (-1)^|m1| / xnlm(il, |m2|), index |m1| * 4 + |m2| */
+} fm_edge;
+
+/* A path list in the atom-index form of feffgpu's forward model (src/feffgpu/exafs/forward.py:143-163, 664-685). */
+typedef struct fm_paths {
+    int32_t npath;
+    int32_t nmax;                /* legs per row of ipots; atoms has nmax - 1 columns */
+    const double *pos;           /* [N][3] atom positions (A) */
+    const int32_t *atoms;        /* [npath][nmax - 1] scatterers in path order */
+    const int32_t *nleg;         /* [npath] 2..9 */
+    const int32_t *absorber;     /* [npath] absorbing atom */
+    const int32_t *ipots;        /* [npath][nmax] potential of the absorber, then of each scatterer */
+    const double *deg;           /* [npath] degeneracy */
+    const int32_t *edge;         /* [npath] index into the edge array, or NULL (all 0) */
+    const double *sigma2;        /* [npath] Debye-Waller sigma^2 (A^2) applied as exp(-2 sigma^2 q^2), or NULL */
+    int32_t cell_kind;           /* 0: cluster, 1: cubic box cell[0], 2: general cell (rows = lattice vectors) */
+    double cell[9];              /* host values */
+    double cellinv[9];           /* host values: inverse of the general cell */
+} fm_paths;
+
+/* The output k grid and its linear-interpolation weights on every edge's phase grid (GPUPathStage.linear_matrix,
+ * src/feffgpu/stage.py:353-363). */
+typedef struct fm_grid {
+    int32_t nk;
+    const double *q;             /* [nk] 1/A */
+    const int32_t *jq;           /* [nedge][nk] lower phase-grid index, clipped to 0..E-2 */
+    const double *fq;            /* [nedge][nk] weight of the upper point */
+} fm_grid;
+
+/* GPUPathStage.cchi (src/feffgpu/stage.py:235-338): npath paths of nleg legs each; coords [npath][nleg][3] (A,
+ * absorber first at the origin), ipots [npath][nleg] (absorber first). Writes cchi [npath][E] complex (interleaved
+ * float for precision 32, double for 64) and reff [npath] (A). */
+int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg, const double *coords,
+            const int64_t *ipots, void *cchi, double *reff, void *stream);
+
+/* Adjoint of fm_cchi: gcchi [npath][E] complex is dL/dRe + i dL/dIm of cchi (same element type as cchi), greff
+ * [npath] dL/dreff or NULL. Writes gcoords [npath][nleg][3] = dL/dcoords. */
+int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg,
+                     const double *coords, const int64_t *ipots, const void *gcchi, const double *greff, double *gcoords,
+                     void *stream);
+
+/* GPUPathStage.chi_fine with the linear ("feff") convention (src/feffgpu/stage.py:365-388): chi [npath][nk] =
+ * A(q) sin(Phi(q) + 2 q reff). cchi as written by fm_cchi; jq/fq of grid are read for edge 0. */
+int fm_chi_fine(const fm_edge *edge, int32_t precision, int32_t npath, const void *cchi, const double *reff,
+                const fm_grid *grid, void *chi, void *stream);
+
+/* Adjoint of fm_chi_fine: gchi [npath][nk] (element type of chi) -> gcchi [npath][E] complex and greff [npath]. */
+int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, const void *cchi, const double *reff,
+                         const fm_grid *grid, const void *gchi, void *gcchi, double *greff, void *stream);
+
+/* The fused path stage over a path list (MultipleScattering.signals / PathSpectra.spectra,
+ * src/feffgpu/exafs/forward.py:143-163, 657-685; chain.run's sum, src/feffgpu/chain.py:45-49):
+ *   chi_path[p][k] = chi of path p on the grid (without the degeneracy, with its Debye-Waller factor),
+ *   chi_seg[s][k] = sum over the paths of segment s, in the order seg_paths lists them, of deg[p] chi_path[p][k].
+ * order [npath] (device) is the processing order, a permutation of 0..npath-1 grouped into nbucket buckets:
+ * bucket b holds order[bucket_offset[b] .. bucket_offset[b+1]) (host array), all of kind bucket_kind[b] (host):
+ * FM_BUCKET_K_MS (K/L1 edge, 3..9 legs), FM_BUCKET_LM_MS (L2/L3/M edges, 3..9 legs), FM_BUCKET_K_SS (K/L1
+ * edge, 2 legs); bucket_nlegmax[b] (host, may be NULL) is the largest nleg in the bucket (sizes LDS).
+ * seg_ptr [nseg + 1] and seg_paths [npath] (device) list the paths of each segment (CSR). chi_path (float for
+ * precision 32, double for 64) is required: it is also the workspace of the deterministic segment sum.
+ * chi_seg (double) and reff (double, A) may be NULL. */
+enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_K_SS = 2 };
+int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
+                 int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,
+                 const int32_t *bucket_kind, const int32_t *bucket_nlegmax, int32_t nseg, const int32_t *seg_ptr,
+                 const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff, void *stream);
+
+/* Adjoint of fm_chi_paths: gseg [nseg][nk] = dL/dchi_seg (double), gpath [npath][nk] = dL/dchi_path (element type
+ * by precision) or NULL. Writes gcoord [npath][nmax][3] (workspace, double) and grad_pos [N][3] = dL/dpos, summed
+ * deterministically per atom over atom_ptr [N + 1] / atom_refs [*] (CSR of (path * nmax + slot) references, slot 0
+ * meaning the absorber). */
+int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
+                          int32_t precision, int32_t iorder, const int32_t *order, const int32_t *segment,
+                          const double *gseg, const void *gpath, int32_t natom, const int32_t *atom_ptr,
+                          const int32_t *atom_refs, double *gcoord, double *grad_pos, void *stream);
+
+/* Message of the last failed call on this thread. */
+const char *fm_last_error(void);
+
+/* Library version string. */
+const char *fm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -21,6 +21,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liborc.so"
 _lib = None
+ROTMODE = {"feff": 0, "exact": 1}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -45,11 +46,12 @@ def lib():
         L.orc_free.argtypes = [ctypes.c_void_p]
         L.orc_info.argtypes = [ctypes.c_void_p, _ip]
         L.orc_tables.argtypes = [ctypes.c_void_p, _ip, _dp, _dp]
-        L.orc_cchi.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _lp, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_cchi.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _lp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_wigner.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]
         L.orc_chi_fine.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_chi_paths.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip,
                                     _ip, _ip, _lp, _dp, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
-                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -88,16 +90,18 @@ class OraclePhase:
         lib().orc_tables(self.h, _p(lm, _ip), _p(xk, _dp), _p(ck, _dp))
         return lm, xk, ck[0::2] + 1j * ck[1::2]
 
-    def cchi(self, coords, ipots, iorder=2):
+    def cchi(self, coords, ipots, iorder=2, rotation="exact"):
         """One path: coords (nleg, 3) A, absorber first at the origin; ipots (nleg,). Returns (cchi (ne1,) complex,
-        reff A, geometry dict with ri (A), beta, eta per leg)."""
+        reff A, geometry dict with ri (A), beta, eta per leg). rotation "exact" (Wigner d through Jacobi
+        polynomials: the batched stage's values) or "feff" (FEFF's rot3i recursion, as in chi.dat)."""
         c = np.ascontiguousarray(coords, dtype=np.float64)
         ip = np.ascontiguousarray(ipots, dtype=np.int64)
         n = c.shape[0]
         out = np.zeros(2 * self.ne1)
         reff = np.zeros(1)
         geom = np.zeros(3 * n)
-        _check(lib().orc_cchi(self.h, n, _p(c, _dp), _p(ip, _lp), int(iorder), _p(out, _dp), _p(reff, _dp), _p(geom, _dp)))
+        _check(lib().orc_cchi(self.h, n, _p(c, _dp), _p(ip, _lp), int(iorder), ROTMODE[rotation], _p(out, _dp), _p(reff, _dp),
+                             _p(geom, _dp)))
         return out[0::2] + 1j * out[1::2], float(reff[0]), dict(ri=geom[:n], beta=geom[n:2 * n], eta=geom[2 * n:])
 
     def chi_fine(self, cchi, reff, q, sigma2=None):
@@ -112,8 +116,14 @@ class OraclePhase:
         return out
 
 
+def wigner_d(l, m1, m2, beta, rotation="exact"):
+    out = np.zeros(1)
+    _check(lib().orc_wigner(ROTMODE[rotation], int(l), int(m1), int(m2), float(beta), _p(out, _dp)))
+    return float(out[0])
+
+
 def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, nseg=1, edge=None, sigma2=None,
-              cell=None, iorder=2, threads=None, want_paths=False):
+              cell=None, iorder=2, threads=None, want_paths=False, rotation="exact"):
     """The fused path-list evaluation in FP64 (see orc_chi_paths in feffref.c). Returns (chi_seg (nseg, NK),
     chi_path (P, NK) or None, reff (P,))."""
     if not isinstance(phases, (list, tuple)):
@@ -144,6 +154,6 @@ def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, n
     nthr = int(threads or os.cpu_count() or 1)
     _check(lib().orc_chi_paths(handles, len(phases), P, nmax, _p(pos, _dp), _p(atoms, _ip), _p(nleg, _ip), _p(absorber, _ip),
                                _p(ipots, _lp), _p(deg, _dp), _p(segment, _ip), _p(edge, _ip), _p(sigma2, _dp), int(nseg),
-                               len(q), _p(q, _dp), _p(cellv, _dp), ncell, int(iorder), nthr, _p(chi_seg, _dp),
+                               len(q), _p(q, _dp), _p(cellv, _dp), ncell, int(iorder), ROTMODE[rotation], nthr, _p(chi_seg, _dp),
                                _p(chi_path, _dp), _p(reff, _dp)))
     return chi_seg, chi_path, reff

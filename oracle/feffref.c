@@ -431,6 +431,56 @@ static void rot3i(int lxp1, int mxp1, double beta, dri_t dri, double *work /* 2 
 #undef D
 }
 
+/* Exact rotation matrices: Wigner's d^j_{m1 m2}(beta) through Jacobi polynomials,
+ *   d = (-1)^lam sqrt(C(2j-k, k+a) / C(k+b, b)) sin(beta/2)^a cos(beta/2)^b P_k^(a,b)(cos beta),
+ *   k = min(j+m2, j-m2, j+m1, j-m1), a, lam by which bound is reached, b = 2j - 2k - a,
+ * in Edmonds' convention (d^1_{10} = -sin(beta)/sqrt 2), the convention of FEFF's rot3i and of the batched stage's
+ * three-term recursion (src/feffgpu/stage.py:41-93). This is what the batched stage computes; FEFF's rot3i
+ * (Edmonds' recursion on truncated matrices) drifts from it above l ~ 20 (rotmode 0 reproduces that). */
+static double jacobi_p(int k, double a, double b, double x) {
+    if (k == 0) return 1.0;
+    double p0 = 1.0, p1 = 0.5 * (a - b + (a + b + 2.0) * x);
+    for (int n = 2; n <= k; ++n) {
+        double s = 2.0 * n + a + b;
+        double c1 = 2.0 * n * (n + a + b) * (s - 2.0);
+        double c2 = (s - 1.0) * (s * (s - 2.0) * x + a * a - b * b);
+        double c3 = 2.0 * (n + a - 1.0) * (n + b - 1.0) * s;
+        double p2 = (c2 * p1 - c3 * p0) / c1;
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+static double lbinom(int n, int k) { return lgamma(n + 1.0) - lgamma(k + 1.0) - lgamma(n - k + 1.0); }
+static double wigner_d(int j, int m1, int m2, double beta) {
+    if (abs(m1) > j || abs(m2) > j) return 0.0;
+    int k = j + m2, a = m1 - m2, lam = m1 - m2;
+    if (j - m2 < k) { k = j - m2; a = m2 - m1; lam = 0; }
+    if (j + m1 < k) { k = j + m1; a = m2 - m1; lam = 0; }
+    if (j - m1 < k) { k = j - m1; a = m1 - m2; lam = m1 - m2; }
+    int b = 2 * j - 2 * k - a;
+    double pref = exp(0.5 * (lbinom(2 * j - k, k + a) - lbinom(k + b, b)));
+    double v = pref * pow(sin(0.5 * beta), a) * pow(cos(0.5 * beta), b) * jacobi_p(k, a, b, cos(beta));
+    return (lam & 1) ? -v : v;
+}
+static void rot_exact(int lxp1, int mxp1, double beta, dri_t dri) {
+    memset(dri, 0, sizeof(dri_t));
+    for (int il = 1; il <= lxp1; ++il) {
+        int mx = il - 1 < mxp1 - 1 ? il - 1 : mxp1 - 1;
+        for (int m1 = -mx; m1 <= mx; ++m1)
+            for (int m2 = -mx; m2 <= mx; ++m2) dri[il][m1 + MTOT + 1][m2 + MTOT + 1] = wigner_d(il - 1, m1, m2, beta);
+    }
+}
+/* for tests: d^l_{m1 m2}(beta) of either algorithm */
+int orc_wigner(int rotmode, int l, int m1, int m2, double beta, double *out) {
+    static __thread dri_t d;
+    static __thread double work[2 * 260 * 260];
+    if (l < 0 || l > 120 || abs(m1) > 3 || abs(m2) > 3) return fail("orc_wigner: l or m out of range");
+    if (rotmode == 0) rot3i(l + 1, 4, beta, d, work); else rot_exact(l + 1, 4, beta, d);
+    *out = d[l + 1][m1 + MTOT + 1][m2 + MTOT + 1];
+    return 0;
+}
+
 /* ---------------------------------------------------------------- sclmz ------------------------------------ */
 #define CLM_M (MTOT + NTOT + 2)
 typedef cplx clm_t[DRI_L][CLM_M];
@@ -549,8 +599,8 @@ static void mmtrxi(const orc_phase *P, orc_work *W, int lam1x, int ie, int ileg,
 
 /* cchi on the first ne1 energies of the phase grid; coords (nleg x 3, A) absorber first at the origin; ipots (nleg)
  * with the absorber first. Returns reff in A. geom (optional, 3 x nleg): ri (A), beta, eta of legs 1..nleg. */
-int orc_cchi(const orc_phase *P, int nleg, const double *coords, const long *ipots, int iorder, double *cchi_out,
-             double *reff_out, double *geom) {
+int orc_cchi(const orc_phase *P, int nleg, const double *coords, const long *ipots, int iorder, int rotmode,
+             double *cchi_out, double *reff_out, double *geom) {
     if (nleg < 2 || nleg > NLEG_MAX) return fail("nleg out of range 2..9");
     int nsc = nleg - 1;
     double rat[NLEG_MAX + 2][3];
@@ -572,7 +622,10 @@ int orc_cchi(const orc_phase *P, int nleg, const double *coords, const long *ipo
     if (!W) return fail("out of memory");
     int size = (2 * P->lmaxp1 + 1 > P->lmaxp1 + S.mmaxp1 ? 2 * P->lmaxp1 + 1 : P->lmaxp1 + S.mmaxp1) + 1;
     W->rwork = malloc(sizeof(double) * 2 * size * size);
-    for (int isc = 1; isc <= nleg; ++isc) rot3i(P->lmaxp1, S.mmaxp1, beta[isc], W->dri[isc], W->rwork);
+    for (int isc = 1; isc <= nleg; ++isc) {
+        if (rotmode == 0) rot3i(P->lmaxp1, S.mmaxp1, beta[isc], W->dri[isc], W->rwork);
+        else rot_exact(P->lmaxp1, S.mmaxp1, beta[isc], W->dri[isc]);
+    }
     /* termination coefficients (mmtr, ipol = 0) */
     static __thread cplx bmati[2 * MTOT + 1][9][2 * MTOT + 1][9];
     memset(bmati, 0, sizeof bmati);
@@ -707,7 +760,7 @@ static void inv3(const double *a, double *o) {
 int orc_chi_paths(const orc_phase *const *phases, int nphase, int npath, int nmax, const double *pos, const int *atoms,
                   const int *nleg, const int *absorber, const long *ipots, const double *deg, const int *segment,
                   const int *edge, const double *sigma2, int nseg, int nq, const double *q, const double *cell, int ncell,
-                  int iorder, int nthreads, double *chi_seg, double *chi_path, double *reff_out) {
+                  int iorder, int rotmode, int nthreads, double *chi_seg, double *chi_path, double *reff_out) {
     double inv[9] = {0};
     if (ncell == 9) inv3(cell, inv);
     int Emax = 0;
@@ -736,7 +789,7 @@ int orc_chi_paths(const orc_phase *const *phases, int nphase, int npath, int nma
                 ip[j] = ipots[(size_t)p * nmax + j];
             }
             double reff;
-            if (orc_cchi(phases[ed], n, coords, ip, iorder, cc, &reff, NULL)) { err = 2; continue; }
+            if (orc_cchi(phases[ed], n, coords, ip, iorder, rotmode, cc, &reff, NULL)) { err = 2; continue; }
             if (reff_out) reff_out[p] = reff;
             chi_fine_one(phases[ed], cc, reff, sigma2 ? sigma2[p] : 0.0, nq, q, chi_all + (size_t)nq * p, amp, pha);
         }
