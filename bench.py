@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the path stage on one MI355X: paths x k-points per second.
+
+Workload (default, "md"): the batched MD-EXAFS case -- a periodic bcc box of 10 x 10 x 10 conventional cells (2000
+atoms, a = 2.87 A), 64 frames of seeded Gaussian displacements (sigma 0.07 A), 16 absorbers per frame, every path of
+2..4 legs with half length <= 5 A (about 2.7e5 paths), output grid k = 0..18 A^-1 step 0.05 (361 points), chi summed
+per frame. Inputs are synthetic. The K-edge phase file is the shipped V K-edge set of FEFF's validation case A018
+(potential 2 = V as the scatterer of every site): no Fe phase file ships with the reference.
+
+A step is one pass of the path stage over the whole batch on the GPU with inputs resident: the fused FP32 kernel
+(FP64 geometry and phase) from positions to chi per frame, and -- when the adjoint is built -- its backward to
+d loss / d positions. Reported beside it: the fused FP64 kernel on the same workload, the roofline of the dominant
+kernel (HIP events on the launch stream), and the FP64 oracle timed on the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload md|fe] [--cpu-seconds 12]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "paths·k-pts/s fwd+bwd (FP32 fused vs FP64 plain, 1 GPU) at R-factor≤2e-5"
+PHASE_K = ROOT / "tests" / "golden" / "validation" / "A018" / "phase.pad"
+
+
+def build_workload(name: str):
+    from feffgpu_amd import workloads as W
+    q = np.round(np.arange(0, 18.0 + 1e-9, 0.05), 10)
+    if name == "md":
+        pos0, box = W.lattice("bcc", 2.87, 10)
+        n = len(pos0)
+        frames, nabs = 64, 16
+        ipot = np.full(n, 2)
+        rng = np.random.default_rng(2024)
+        pos_all, lists = [], []
+        for f in range(frames):
+            pos = W.displaced(pos0, 0.07, (2024, f))
+            pos_all.append(pos)
+            for a in rng.choice(n, nabs, replace=False):
+                lists.append(W.enumerate_paths(pos, int(a), ipot, 5.0, 4, box=box, offset=f * n, segment=f, nmax=4))
+        pl = W.concat(lists, 4)
+        cfg = dict(workload="md-batch bcc 2000 atoms x 64 frames, 16 absorbers/frame, nleg 2-4, Rmax 5 A, k 0-18/0.05",
+                   atoms=n, frames=frames, absorbers_per_frame=nabs, rmax_A=5.0, nleg_max=4, nk=len(q),
+                   phase="FEFF validation A018 V K edge (substitute for Fe K)", cell_A=float(box[0]))
+        return np.concatenate(pos_all), pl, q, float(box[0]), frames, cfg
+    if name == "fe":
+        cl = W.cluster("bcc", 2.87, 9.0)
+        pos = W.displaced(cl, 0.05, 7, keep_first=True)
+        pl = W.enumerate_paths(pos, 0, np.full(len(pos), 2), 6.0, 4, nmax=4)
+        cfg = dict(workload="bcc cluster 9 A (259 atoms), sigma 0.05 A, nleg <= 4, Rmax 6 A, k 0-18/0.05",
+                   atoms=len(pos), rmax_A=6.0, nleg_max=4, nk=len(q), phase="FEFF validation A018 V K edge")
+        return pos, pl, q, None, 1, cfg
+    raise SystemExit(f"unknown workload {name}")
+
+
+def timed(fn, steps, warmup, sync):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(pos, pl, q, cell, seconds):
+    """The FP64 oracle (OpenMP over paths) on a bounded sample of the same path list."""
+    import oracle
+    ph = oracle.OraclePhase(PHASE_K)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or threads))
+    rng = np.random.default_rng(5)
+
+    def run(idx):
+        t0 = time.perf_counter()
+        oracle.chi_paths(ph, pos, pl.atoms[idx], pl.nleg[idx], pl.absorber[idx], pl.ipots[idx], pl.deg[idx], q,
+                         cell=cell, threads=threads)
+        return time.perf_counter() - t0
+
+    probe = np.sort(rng.choice(len(pl), min(len(pl), 256 * threads), replace=False))
+    t = run(probe)
+    n = int(min(len(pl), max(len(probe), len(probe) * seconds / max(t, 1e-3))))
+    idx = np.sort(rng.choice(len(pl), n, replace=False))
+    t = run(idx)
+    return dict(value=n * len(q) / t, unit="paths·k-pts/s", cores=threads, kind="port",
+                sample=f"{n} of {len(pl)} paths drawn at random (seed 5), all {len(q)} k-points, FP64 oracle "
+                       f"(oracle/feffref.c, OpenMP), {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="md")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py measures one GPU: this build runs the path stage on a single MI355X (no multi-GPU path)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from feffgpu_amd.paths import PathPlan
+    from feffgpu_amd.roofline import FLOPS_PER_CMAC, MI355X_HBM_TBPS, MI355X_PEAK_TFLOPS, path_cmacs
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a ROCm GPU")
+    dev = torch.device("cuda", 0)
+    t_setup = time.perf_counter()
+    pos, pl, q, cell, nseg, cfg = build_workload(args.workload)
+    print(f"[bench] workload {args.workload}: {len(pl)} paths, {len(q)} k-points, setup {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
+    common = dict(segment=pl.segment, nseg=nseg, cell=cell, device=dev)
+    plan32 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
+    plan64 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, **common)
+    pos_t = torch.tensor(pos, device=dev, dtype=torch.float64)
+    sync = torch.cuda.synchronize
+    units = len(pl) * len(q)
+
+    # parity of this workload: FP32 fused against FP64 fused (equal to the FP64 oracle to ~1e-14)
+    s32 = plan32.forward(pos_t)[0].cpu().numpy()
+    s64 = plan64.forward(pos_t)[0].cpu().numpy()
+    from feffgpu_amd.compare import r_factor
+    Rs = [r_factor(q, s64[i], s32[i], kmin=3.0, kmax=17.0, rmin=1.0, rmax=5.0) for i in range(nseg)]
+    maxrel = float(np.abs(s32 - s64).max() / np.abs(s64).max())
+
+    ms_fwd32 = timed(lambda: plan32.forward(pos_t), args.steps, args.warmup, sync) * 1e3
+    print(f"[bench] fp32 fused forward {ms_fwd32:.3f} ms/step", file=sys.stderr, flush=True)
+    ms_fwd64 = timed(lambda: plan64.forward(pos_t), max(2, args.steps // 2), 1, sync) * 1e3
+    print(f"[bench] fp64 fused forward {ms_fwd64:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel: the bucket with the most work, timed alone with HIP events on its stream
+    cm = path_cmacs(plan32.tables[0], plan32.nleg_h, pl.ipots)
+    b_work = []
+    for b in range(len(plan32.buckets.kind)):
+        sel = plan32.buckets.order[plan32.buckets.offset[b]:plan32.buckets.offset[b + 1]]
+        b_work.append(cm[sel].sum() * FLOPS_PER_CMAC)
+    bdom = int(np.argmax(b_work))
+    stream = torch.cuda.current_stream(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    nrep = max(3, args.steps)
+    for _ in range(2):
+        plan32.forward_bucket(pos_t, bdom)
+    ev[0].record(stream)
+    for _ in range(nrep):
+        plan32.forward_bucket(pos_t, bdom)
+    ev[1].record(stream)
+    sync()
+    t_dom = ev[0].elapsed_time(ev[1]) / nrep * 1e-3
+    achieved = b_work[bdom] / t_dom / 1e12
+    kind_names = {0: "K multiple scattering (3-4 legs)", 1: "L/M multiple scattering", 2: "K single scattering"}
+    roof = dict(bound="mfma", achieved=achieved, peak=MI355X_PEAK_TFLOPS["fp32"], unit="TFLOP/s",
+                frac=achieved / MI355X_PEAK_TFLOPS["fp32"], traffic=None,
+                kernel=f"k_forward<float,{int(plan32.buckets.kind[bdom])}> ({kind_names.get(int(plan32.buckets.kind[bdom]))})",
+                launch_ms=t_dom * 1e3, flops_per_launch=b_work[bdom],
+                note="FP32 vector pipes (no MFMA); 157.3 is MI355X's FP32 peak with packed math, which this build "
+                     "disables (non-packed VALU ceiling 78.6)")
+
+    out = dict(metric=METRIC, value=units / (ms_fwd32 * 1e-3), unit="paths·k-pts/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=ms_fwd32, higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="fp32 (fp64 geometry and phase)", data="synthetic",
+               config=dict(cfg, paths=len(pl), parallelism="1 GPU"),
+               mode="forward (adjoint not built yet)",
+               fp64_fused_fwd=dict(value=units / (ms_fwd64 * 1e-3), ms_per_step=ms_fwd64),
+               parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel),
+               roofline=roof)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pos, pl, q, cell, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -297,4 +297,201 @@ __device__ __forceinline__ double dseed(int l, int m, int n, double c, double s,
     return sign * v;
 }
 
+// d/dbeta of the seeds above (same symmetry relations, which are linear).
+__device__ __forceinline__ double dseed_db(int l, int m, int n, double c, double s, double c2, double s2) {
+    if (l == 0) return 0.0;
+    if (l == 1) {
+        if (m < -1 || m > 1 || n < -1 || n > 1) return 0.0;
+        const double r2 = 0.70710678118654752440;
+        if ((m == 1 && n == 1) || (m == -1 && n == -1)) return -0.5 * s;
+        if ((m == 1 && n == -1) || (m == -1 && n == 1)) return 0.5 * s;
+        if (m == 0 && n == 0) return -s;
+        if ((m == 1 && n == 0) || (m == 0 && n == -1)) return -c * r2;
+        return c * r2;   // (0, 1), (-1, 0)
+    }
+    if (m < -2 || m > 2 || n < -2 || n > 2) return 0.0;
+    double sign = 1.0;
+    int am = m, an = n;
+    if (abs(an) > abs(am)) {
+        int t = am; am = an; an = t;
+        if ((m - n) & 1) sign = -sign;
+    }
+    if (am < 0) {
+        am = -am; an = -an;
+        if ((m - n) & 1) sign = -sign;
+    }
+    double v;
+    if (am == 2) {
+        if (an == 2) v = -2.0 * c2 * c2 * c2 * s2;
+        else if (an == 1) v = 0.5 * s * s - 0.5 * (1 + c) * c;
+        else if (an == 0) v = 2.0 * 0.61237243569579452455 * s * c;
+        else if (an == -1) v = -0.5 * s * s - 0.5 * (1 - c) * c;
+        else v = 2.0 * s2 * s2 * s2 * c2;
+    } else if (am == 1) {
+        if (an == 1) v = -0.5 * s * (4 * c + 1);
+        else if (an == 0) v = -1.2247448713915890491 * (c * c - s * s);
+        else v = 0.5 * s * (4 * c - 1);
+    } else {
+        v = -3.0 * c * s;
+    }
+    return sign * v;
+}
+
+// ------------------------------------------------------------------------------------------------ dual numbers
+// Forward-mode derivative of the FP64 leg geometry along one coordinate direction, with the value branches of the
+// regularised geometry (trig / arg / clamp) taken as in the forward: the derivative of a constant branch is zero.
+struct Dd { double v, d; };
+__device__ __forceinline__ Dd dd(double v, double d = 0.0) { return Dd{v, d}; }
+__device__ __forceinline__ Dd operator+(Dd a, Dd b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dd operator-(Dd a, Dd b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dd operator-(Dd a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dd operator*(Dd a, Dd b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dd operator/(Dd a, Dd b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ __forceinline__ Dd dsqrt(Dd a) {
+    const double s = sqrt(a.v);
+    return {s, s > 0 ? a.d / (2.0 * s) : 0.0};
+}
+__device__ __forceinline__ Dd datan2(Dd y, Dd x) {
+    const double q = x.v * x.v + y.v * y.v;
+    return {atan2(y.v, x.v), q > 0 ? (x.v * y.d - y.v * x.d) / q : 0.0};
+}
+
+struct TrigD { Dd ct, st, cp, sp; };
+__device__ __forceinline__ TrigD trig_dual(Dd x, Dd y, Dd z) {
+    TrigD t;
+    Dd r = dsqrt(x * x + y * y + z * z), rxy = dsqrt(x * x + y * y);
+    if (r.v < 1e-6) { t.ct = dd(1.0); t.st = dd(0.0); } else { t.ct = z / r; t.st = rxy / r; }
+    if (rxy.v < 1e-6) { t.cp = dd(t.ct.v < 0 ? -1.0 : 1.0); t.sp = dd(0.0); } else { t.cp = x / rxy; t.sp = y / rxy; }
+    return t;
+}
+__device__ __forceinline__ Dd arg_dual(Dd x, Dd y, Dd fi) {
+    if (fabs(x.v) < 1e-6) x = dd(0.0);
+    if (fabs(y.v) < 1e-6) y = dd(0.0);
+    return (x.v == 0.0 && y.v == 0.0) ? fi : datan2(y, x);
+}
+struct LegGeoD { Dd r, beta, alpha, gamma; };
+__device__ __forceinline__ LegGeoD leg_geometry_dual(const Dd (*rat)[3], int J, int n) {
+    int i, ip1, im1;
+    if (J == n) { i = 0; ip1 = 1; im1 = n - 1; } else { i = J; ip1 = J + 1; im1 = J - 1; }
+    Dd vo[3], vi[3];
+    for (int c = 0; c < 3; ++c) {
+        vo[c] = rat[ip1][c] - rat[i][c];
+        vi[c] = rat[i][c] - rat[im1][c];
+    }
+    TrigD p = trig_dual(vo[0], vo[1], vo[2]);
+    TrigD q = trig_dual(vi[0], vi[1], vi[2]);
+    Dd cppp = q.cp * p.cp + q.sp * p.sp, sppp = p.sp * q.cp - p.cp * q.sp;
+    Dd phi = datan2(q.sp, q.cp), phip = datan2(p.sp, p.cp);
+    Dd ar = -(q.st * p.ct - q.ct * p.st * cppp), ai = p.st * sppp;
+    Dd gr = -(q.st * p.ct * cppp - q.ct * p.st), gi = -(q.st * sppp);
+    Dd b = q.ct * p.ct + q.st * p.st * cppp;
+    if (b.v > 1.0) b = dd(1.0);
+    if (b.v < -1.0) b = dd(-1.0);
+    Dd u1x = q.st * q.cp, u1y = q.st * q.sp, u1z = q.ct;
+    Dd u2x = p.st * p.cp, u2y = p.st * p.sp, u2z = p.ct;
+    Dd cx_ = u1y * u2z - u1z * u2y, cy_ = u1z * u2x - u1x * u2z, cz_ = u1x * u2y - u1y * u2x;
+    LegGeoD g;
+    g.beta = datan2(dsqrt(cx_ * cx_ + cy_ * cy_ + cz_ * cz_ + dd(1e-30)), b);
+    Dd a_j = arg_dual(ar, ai, phip - phi), g_j = arg_dual(gr, gi, dd(0.0));
+    g.alpha = dd(M_PI) - g_j;
+    g.gamma = dd(M_PI) - a_j;
+    g.r = dsqrt(vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
+    return g;
+}
+
+// ------------------------------------------------------------------------------------------------ Hankel tangents
+// The spherical-wave factors and their derivative with respect to the leg length r (z = -i/(ck r), dz/dr = -z/r),
+// the recursions of Hank3/Hank2 differentiated row by row.
+template <class T> struct Hank3T {
+    cx<T> z, zd, a2, a1, b2, b1, c2, c1, a2d, a1d, b2d, b1d, c2d, c1d;
+    __device__ __forceinline__ void init(cx<T> z_, T r) {
+        z = z_;
+        zd = z * (T(-1) / r);
+        a2 = a1 = b2 = b1 = c2 = c1 = a2d = a1d = b2d = b1d = c2d = c1d = mk<T>(T(0), T(0));
+    }
+    __device__ __forceinline__ void shift(const cx<T> (&o)[3], const cx<T> (&od)[3]) {
+        a2 = a1; a1 = o[0]; b2 = b1; b1 = o[1]; c2 = c1; c1 = o[2];
+        a2d = a1d; a1d = od[0]; b2d = b1d; b1d = od[1]; c2d = c1d; c1d = od[2];
+    }
+    __device__ __forceinline__ void seed_row(int r, cx<T> (&o)[3], cx<T> (&od)[3], bool alive = true) {
+        const cx<T> one = mk<T>(T(1), T(0)), zero = mk<T>(T(0), T(0));
+        if (r == 1) { o[0] = one; o[1] = zero; o[2] = zero; od[0] = od[1] = od[2] = zero; }
+        else if (r == 2) {
+            o[0] = one - z; o[1] = mk<T>(-z.r, -z.i); o[2] = zero;
+            od[0] = mk<T>(-zd.r, -zd.i); od[1] = od[0]; od[2] = zero;
+        } else if (r == 3) {
+            o[0] = a2 - (z * T(3)) * a1;
+            od[0] = a2d - (zd * a1 + z * a1d) * T(3);
+            o[1] = (z * T(-3)) * (one - z * T(2));
+            od[1] = zd * T(-3) + (z * zd) * T(12);
+            o[2] = z * z * T(3);
+            od[2] = (z * zd) * T(6);
+        } else {
+            o[0] = a2 - (z * T(5)) * a1;
+            od[0] = a2d - (zd * a1 + z * a1d) * T(5);
+            o[1] = b2 - (z * T(5)) * (b1 + a1);
+            od[1] = b2d - (zd * (b1 + a1) + z * (b1d + a1d)) * T(5);
+            o[2] = (z * z * T(15)) * (one - z * T(3));
+            od[2] = (z * zd) * T(30) - (z * z * zd) * T(135);
+        }
+        if (!alive) {
+            if (r > 2) { o[0] = zero; od[0] = zero; }
+            o[1] = o[2] = od[1] = od[2] = zero;
+        }
+        shift(o, od);
+    }
+    __device__ __forceinline__ void step(int r, cx<T> (&o)[3], cx<T> (&od)[3]) {
+        const T k = T(2 * r - 3);
+        const cx<T> zt = z * k, ztd = zd * k;
+        const cx<T> s0 = a1, s1 = b1 + a1, s2 = c1 + b1;
+        const cx<T> s0d = a1d, s1d = b1d + a1d, s2d = c1d + b1d;
+        o[0] = a2 - zt * s0;
+        o[1] = b2 - zt * s1;
+        o[2] = c2 - zt * s2;
+        od[0] = a2d - (ztd * s0 + zt * s0d);
+        od[1] = b2d - (ztd * s1 + zt * s1d);
+        od[2] = c2d - (ztd * s2 + zt * s2d);
+        shift(o, od);
+    }
+};
+template <class T> struct Hank2T {
+    cx<T> z, zd, a2, a1, b2, b1, a2d, a1d, b2d, b1d;
+    __device__ __forceinline__ void init(cx<T> z_, T r) {
+        z = z_;
+        zd = z * (T(-1) / r);
+        a2 = a1 = b2 = b1 = a2d = a1d = b2d = b1d = mk<T>(T(0), T(0));
+    }
+    __device__ __forceinline__ void shift(const cx<T> (&o)[2], const cx<T> (&od)[2]) {
+        a2 = a1; a1 = o[0]; b2 = b1; b1 = o[1];
+        a2d = a1d; a1d = od[0]; b2d = b1d; b1d = od[1];
+    }
+    __device__ __forceinline__ void seed_row(int r, cx<T> (&o)[2], cx<T> (&od)[2], bool alive = true) {
+        const cx<T> one = mk<T>(T(1), T(0)), zero = mk<T>(T(0), T(0));
+        if (r == 1) { o[0] = one; o[1] = zero; od[0] = od[1] = zero; }
+        else if (r == 2) { o[0] = one - z; o[1] = mk<T>(-z.r, -z.i); od[0] = mk<T>(-zd.r, -zd.i); od[1] = od[0]; }
+        else {
+            o[0] = a2 - (z * T(3)) * a1;
+            od[0] = a2d - (zd * a1 + z * a1d) * T(3);
+            o[1] = (z * T(-3)) * (one - z * T(2));
+            od[1] = zd * T(-3) + (z * zd) * T(12);
+        }
+        if (!alive) {
+            if (r > 2) { o[0] = zero; od[0] = zero; }
+            o[1] = od[1] = zero;
+        }
+        shift(o, od);
+    }
+    __device__ __forceinline__ void step(int r, cx<T> (&o)[2], cx<T> (&od)[2], bool alive = true) {
+        const T k = T(2 * r - 3);
+        const cx<T> zt = z * k, ztd = zd * k;
+        const cx<T> s0 = a1, s1 = b1 + a1, s0d = a1d, s1d = b1d + a1d;
+        o[0] = a2 - zt * s0;
+        o[1] = b2 - zt * s1;
+        od[0] = a2d - (ztd * s0 + zt * s0d);
+        od[1] = b2d - (ztd * s1 + zt * s1d);
+        if (!alive) { o[0] = o[1] = od[0] = od[1] = mk<T>(T(0), T(0)); }
+        shift(o, od);
+    }
+};
+
 }  // namespace fm

@@ -1,0 +1,429 @@
+// Per-path machinery shared by the forward and adjoint kernels: per-wave LDS layout, path loading and FP64 leg
+// geometry, rotation tables, the streamed scattering-matrix blocks, the amplitude at one energy and the fine-grid
+// epilogue. Reference semantics: src/feffgpu/stage.py:147-388, src/feffgpu/exafs/forward.py:153-163.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "fm_device.h"
+
+namespace fm {
+
+constexpr int FM_MAX_EDGE = 4;
+constexpr int WPB_MAX = 4;        // wavefronts (paths) per workgroup at most
+constexpr int FM_MAX_ECHUNK = 4;  // energies up to 4 x 64
+#define FM_WPE(T) (sizeof(T) == 4 ? 2 : 1)   // wavefronts per SIMD the register budget is sized for
+
+// Where a kernel reads its paths: mode 0 explicit coordinates (GPUPathStage.cchi), mode 1 a path list.
+struct PathSrc {
+    int mode;
+    int nleg;                 // mode 0
+    const double *coords;     // mode 0: [npath][nleg][3] A
+    const int64_t *ipots64;   // mode 0: [npath][nleg]
+    fm_paths P;               // mode 1
+};
+
+// ------------------------------------------------------------------------------------------------ per-wave LDS
+struct PathLDS {
+    double rat[FM_MAX_LEGS + 1][3];   // bohr, rat[0] = absorber (origin), rat[1..n-1] scatterers
+    double r[FM_MAX_LEGS + 1], beta[FM_MAX_LEGS + 1], alpha[FM_MAX_LEGS + 1], gamma[FM_MAX_LEGS + 1];
+    double eta[FM_MAX_LEGS + 1];
+    double c1[FM_MAX_LEGS + 1], s1[FM_MAX_LEGS + 1], c2[FM_MAX_LEGS + 1], s2[FM_MAX_LEGS + 1];  // cos/sin eta, 2 eta
+    int ipot[FM_MAX_LEGS + 1];       // potential of path atom j (0 and n: absorber)
+    double reff, R, prod_r;           // bohr
+    int n, pad_[3];
+};
+
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+template <class T> __host__ __device__ inline size_t wave_lds_bytes(int nlegmax, int ltab, int emax) {
+    return align16(sizeof(PathLDS)) + align16(sizeof(T) * (size_t)nlegmax * ltab * FM_NREP) +
+           align16(sizeof(T) * (size_t)emax) + align16(sizeof(double) * (size_t)emax);
+}
+
+// ------------------------------------------------------------------------------------------------ setup of one path
+// Fills g (geometry, potentials) for path slot `p`; all lanes call it. Returns n.
+__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx) {
+    const int lane = lane_id();
+    int n;
+    if (a.mode == 0) {
+        n = a.nleg;
+        edge_idx = 0;
+        if (lane < n) {
+            const double *c = a.coords + ((size_t)p * n + lane) * 3;
+            double x = c[0] / FM_BOHR, y = c[1] / FM_BOHR, z = c[2] / FM_BOHR;
+            // absorber (lane 0) is the path's first point; the stage places it at rat[0] and rat[n]
+            g.rat[lane][0] = x; g.rat[lane][1] = y; g.rat[lane][2] = z;
+            g.ipot[lane] = (int)a.ipots64[(size_t)p * n + lane];
+        }
+    } else {
+        n = a.P.nleg[p];
+        edge_idx = a.P.edge ? a.P.edge[p] : 0;
+        const int nmax = a.P.nmax;
+        if (lane < n) {
+            double v[3] = {0.0, 0.0, 0.0};
+            if (lane > 0) {
+                const int at = a.P.atoms[(size_t)p * (nmax - 1) + lane - 1];
+                const int ab = a.P.absorber[p];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[c] = a.P.pos[3 * (size_t)at + c] - a.P.pos[3 * (size_t)ab + c];
+                if (a.P.cell_kind == 1) {
+                    const double L = a.P.cell[0];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[c] -= L * rint(v[c] / L);
+                } else if (a.P.cell_kind == 2) {
+                    double f[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        f[c] = v[0] * a.P.cellinv[0 * 3 + c] + v[1] * a.P.cellinv[1 * 3 + c] + v[2] * a.P.cellinv[2 * 3 + c];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) f[c] -= rint(f[c]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        v[c] = f[0] * a.P.cell[0 * 3 + c] + f[1] * a.P.cell[1 * 3 + c] + f[2] * a.P.cell[2 * 3 + c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g.rat[lane][c] = v[c] / FM_BOHR;
+            g.ipot[lane] = a.P.ipots[(size_t)p * nmax + lane];
+        }
+    }
+    wave_sync();
+    if (lane < 3) g.rat[n][lane] = g.rat[0][lane];
+    if (lane == 0) { g.ipot[n] = g.ipot[0]; g.n = n; }
+    wave_sync();
+    // one lane per leg
+    if (lane < n) {
+        LegGeo lg = leg_geometry(g.rat, lane + 1, n);
+        g.r[lane + 1] = lg.r;
+        g.beta[lane + 1] = lg.beta;
+        g.alpha[lane + 1] = lg.alpha;
+        g.gamma[lane + 1] = lg.gamma;
+    }
+    wave_sync();
+    if (lane < n) {
+        const int J = lane + 1;
+        double eta = (J == 1 ? g.alpha[n] : g.alpha[J - 1]) + g.gamma[J];
+        g.eta[J] = eta;
+        double s, c;
+        sincos(eta, &s, &c);
+        g.c1[J] = c; g.s1[J] = s;
+        sincos(2.0 * eta, &s, &c);
+        g.c2[J] = c; g.s2[J] = s;
+    }
+    if (lane == 0) {
+        double R = 0.0, pr = 1.0;
+        for (int j = 1; j <= n; ++j) { R += g.r[j]; pr *= g.r[j]; }
+        g.R = R; g.reff = 0.5 * R; g.prod_r = pr;
+    }
+    wave_sync();
+    return n;
+}
+
+// Rotation tables: W[s][il][k] = d^{il-1}_{m1 m2}(beta_{s+1}) xnlm(il,|m1|)(-1)^|m1| / xnlm(il,|m2|) for legs
+// s = 0..n-1 and the FM_NREP representatives k, il = 1..L1 (row 0 unused). Three-term recursion in l for fixed
+// (m1, m2) (src/feffgpu/stage.py:77-92), seeded by the closed forms at l <= 2, in FP64.
+template <class T, int SET>
+__device__ void build_rotation(const fm_edge &ed, const PathLDS &g, int n, int ltab, T *W) {
+    constexpr PairDef PD = pairdef(SET);
+    const int lane = lane_id();
+    const int ntask = n * FM_NREP;
+    for (int t = lane; t < ntask; t += FM_WAVE) {
+        const int s = t / FM_NREP, k = t % FM_NREP;
+        int m1 = 0, m2 = 0;
+#pragma unroll
+        for (int q = 0; q < FM_NREP; ++q)
+            if (q == k) { m1 = PD.m1[q]; m2 = PD.m2[q]; }
+        const double be = g.beta[s + 1];
+        double c = cos(be), sn = sin(be), c2 = cos(0.5 * be), s2 = sin(0.5 * be);
+        T *Ws = W + (size_t)s * ltab * FM_NREP;
+        const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
+        const int mm = (m1 + 3) * 7 + (m2 + 3), nn = am1 * 4 + am2;
+        double dlm1 = 0.0, dl = 0.0;
+        for (int il = 1; il <= ed.L1; ++il) {
+            const int l = il - 1;
+            double d;
+            if (l <= 2) {
+                d = dseed(l, m1, m2, c, sn, c2, s2);
+            } else {
+                const double *co = ed.rotc + ((size_t)(l - 1) * 49 + mm) * 3;   // from l - 1 to l
+                d = (co[0] * c - co[1]) * dl - co[2] * dlm1;
+            }
+            dlm1 = dl;
+            dl = d;
+            Ws[(size_t)il * FM_NREP + k] = (T)(d * ed.rotn[(size_t)il * 16 + nn]);
+        }
+    }
+    wave_sync();
+}
+
+// One partial-wave row of a scattering matrix: Ft[k] += X_a(k) Y_b(k) w[k] for the representatives k of the
+// NR x NC block (Y already carries the t-matrix).
+template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void accum_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&xa)[3], cx<T> y0, cx<T> y1,
+                                          const T *__restrict__ w) {
+    constexpr PairDef PD = pairdef(SET);
+    cx<T> Z[3][2];
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa) {
+        Z[aa][0] = xa[aa] * y0;
+        Z[aa][1] = xa[aa] * y1;
+    }
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k)
+        if (PD.ri[k] < NR && PD.rj[k] < NC) rfma(Ft[k], Z[PD.a[k] - 1][PD.b[k] - 1], w[k]);
+}
+
+// F_s of the path at the lane's energy: X = Hankel factors of leg s+1 (gam), Y = of leg s (gamtl) times the
+// scatterer's t-matrix, summed over il <= ilx with the rotation table of leg s (src/feffgpu/stage.py:287-305).
+// Accumulates the representatives k with ri < NR, rj < NC into Ft (zeroed here). Rows up to ilmax (the wave's
+// largest ilx) are generated; a lane adds only its rows il <= ilx.
+template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T rY, int ilx, int ilmax,
+                                       const T *__restrict__ Wl, const T *__restrict__ tl, int tlstride) {
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k) Ft[k] = mk<T>(T(0), T(0));
+    Hank3<T> X;
+    Hank2<T> Y;
+    X.init(minus_i_over(ck * rX));
+    Y.init(minus_i_over(ck * rY));
+    cx<T> xa[3], yb[2];
+    const int rseed = ilmax < 4 ? ilmax : 4;
+    for (int r = 1; r <= rseed; ++r) {
+        X.seed_row(r, xa);
+        if (r < 4) Y.seed_row(r, yb); else Y.step(r, yb);
+        const cx<T> t = mk<T>(tl[(size_t)r * tlstride], tl[(size_t)r * tlstride + 1]);
+        if (r <= ilx) accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
+    }
+#pragma unroll 1
+    for (int r = 5; r <= ilmax; ++r) {
+        X.step(r, xa);
+        Y.step(r, yb);
+        const cx<T> t = mk<T>(tl[(size_t)r * tlstride], tl[(size_t)r * tlstride + 1]);
+        const T *w = Wl + (size_t)r * FM_NREP;
+        if (r <= ilx) accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, w);
+    }
+}
+
+// The row phases exp(-i eta m) for m = M[i].
+template <class T, int SET>
+__device__ __forceinline__ void row_phases(cx<T> (&ph)[6], const PathLDS &g, int J) {
+    constexpr LamDef D = lamdef(SET);
+    const T c1 = (T)g.c1[J], s1 = (T)g.s1[J], c2 = (T)g.c2[J], s2 = (T)g.s2[J];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const int m = D.M[i];
+        ph[i] = m == 0 ? mk<T>(T(1), T(0))
+                : m == 1 ? mk<T>(c1, -s1)
+                : m == -1 ? mk<T>(c1, s1)
+                : m == 2 ? mk<T>(c2, -s2)
+                          : mk<T>(c2, s2);
+    }
+}
+
+template <class T> struct TL;
+template <> struct TL<float> {
+    static __device__ __forceinline__ const float *tl(const fm_edge &e) { return e.tl32; }
+    static __device__ __forceinline__ const float *chw(const fm_edge &e) { return e.chw32; }
+};
+template <> struct TL<double> {
+    static __device__ __forceinline__ const double *tl(const fm_edge &e) { return e.tl64; }
+    static __device__ __forceinline__ const double *chw(const fm_edge &e) { return e.chw64; }
+};
+
+// cchi at energy e of the path held in g (tables in W). All lanes of the wave run it (e may be out of range for
+// some; they compute garbage that the caller drops, which keeps the loops uniform).
+template <class T, int SET>
+__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n) {
+    constexpr LamDef D = lamdef(SET);
+    constexpr PairDef PD = pairdef(SET);
+    constexpr int L = D.L, L0 = D.L0;
+    const int E = ed.E;
+    const int ee = e < E ? e : E - 1;
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    const int tlstride = 2 * E;
+    const T *tlbase = TL<T>::tl(ed);
+    auto lmaxA = [&](int j) -> int { return ed.lmax[(size_t)g.ipot[j] * E + ee]; };
+
+    // ---- termination T (mmtrxi + mmtr, src/feffgpu/stage.py:318-334): X = leg 1, Y = leg n, channel rows only
+    cx<T> U[L0][L];
+    {
+        const int lx1 = max(lmaxA(0), lmaxA(1)) + 1;          // leg 1: absorber -> atom 1
+        const int lxn = max(lmaxA(n - 1), lmaxA(n)) + 1;      // leg n: atom n-1 -> absorber
+        Hank3<T> X;
+        Hank2<T> Y;
+        X.init(minus_i_over(ck * (T)g.r[1]));
+        Y.init(minus_i_over(ck * (T)g.r[n]));
+        cx<T> Tt[FM_NREP];
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) Tt[k] = mk<T>(T(0), T(0));
+        const T *Wn = W + (size_t)(n - 1) * ltab * FM_NREP;
+        const T *chw = TL<T>::chw(ed);
+        cx<T> xa[3], yb[2];
+        for (int r = 1; r <= 4; ++r) {
+            X.seed_row(r, xa, r <= lx1);
+            if (r < 4) Y.seed_row(r, yb, r <= lxn); else Y.step(r, yb, r <= lxn);
+            int c = -1;
+            for (int q = 0; q < ed.nchan; ++q)
+                if (ed.chan_il[q] == r) c = q;
+            if (c >= 0) {
+                const cx<T> w = mk<T>(chw[((size_t)c * E + ee) * 2], chw[((size_t)c * E + ee) * 2 + 1]) * T(2 * r - 1);
+                accum_row<T, SET, L0, L0>(Tt, xa, yb[0] * w, yb[1] * w, Wn + (size_t)r * FM_NREP);
+            }
+        }
+        cx<T> ph[6];
+        row_phases<T, SET>(ph, g, 1);
+#pragma unroll
+        for (int i = 0; i < L0; ++i)
+#pragma unroll
+            for (int j = 0; j < L; ++j)
+                U[i][j] = (j < L0) ? ph[i] * Tt[PD.idx[i][j]] * T(PD.sg[i][j]) : mk<T>(T(0), T(0));
+    }
+
+    cx<T> ptrac = mk<T>(T(0), T(0));
+    cx<T> Ft[FM_NREP];
+    cx<T> ph[6];
+    auto scat = [&](int s, int &ilx, int &ilmax, const T *&tl, const T *&Wl) {
+        const int ip = g.ipot[s];
+        ilx = ed.lmax[(size_t)ip * E + ee] + 1;
+        ilmax = ed.lmax1[ip];
+        tl = tlbase + ((size_t)ip * ltab) * tlstride + 2 * ee;
+        Wl = W + (size_t)(s - 1) * ltab * FM_NREP;
+    };
+    // U <- U F with F = diag(ph) Ft (expanded), row by row of U; U has NCU valid columns (rows of F)
+    auto chain = [&](auto ncu_tag) {
+        constexpr int NCU = decltype(ncu_tag)::value;
+#pragma unroll
+        for (int a = 0; a < L0; ++a) {
+            cx<T> row[L];
+#pragma unroll
+            for (int i = 0; i < L; ++i) row[i] = (i < NCU) ? U[a][i] * ph[i] : mk<T>(T(0), T(0));
+#pragma unroll
+            for (int j = 0; j < L; ++j) {
+                cx<T> acc = mk<T>(T(0), T(0));
+#pragma unroll
+                for (int i = 0; i < NCU; ++i) {
+                    const cx<T> f = Ft[PD.idx[i][j]];
+                    if (PD.sg[i][j] > 0) cfma(acc, row[i], f);
+                    else cfma(acc, row[i], mk<T>(-f.r, -f.i));
+                }
+                U[a][j] = acc;
+            }
+        }
+    };
+    if (n == 2) {
+        int ilx, ilmax;
+        const T *tl, *Wl;
+        scat(1, ilx, ilmax, tl, Wl);
+        fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, ilmax, Wl, tl, tlstride);
+        row_phases<T, SET>(ph, g, 2);
+#pragma unroll
+        for (int a = 0; a < L0; ++a)
+#pragma unroll
+            for (int b = 0; b < L0; ++b) {
+                const cx<T> f = Ft[PD.idx[b][a]] * T(PD.sg[b][a]);
+                cfma(ptrac, U[a][b] * ph[b], f);
+            }
+    } else {
+        // s = n - 1: F is L0 x L; U (L0 x L0) -> U F (L0 x L)
+        {
+            int ilx, ilmax;
+            const T *tl, *Wl;
+            scat(n - 1, ilx, ilmax, tl, Wl);
+            fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, ilmax, Wl, tl, tlstride);
+            row_phases<T, SET>(ph, g, n);
+            chain(std::integral_constant<int, L0>{});
+        }
+#pragma unroll 1
+        for (int s = n - 2; s >= 2; --s) {
+            int ilx, ilmax;
+            const T *tl, *Wl;
+            scat(s, ilx, ilmax, tl, Wl);
+            fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, ilmax, Wl, tl, tlstride);
+            row_phases<T, SET>(ph, g, s + 1);
+            chain(std::integral_constant<int, L>{});
+        }
+        // s = 1: F is L x L0; trace of U (L0 x L) F (L x L0)
+        {
+            int ilx, ilmax;
+            const T *tl, *Wl;
+            scat(1, ilx, ilmax, tl, Wl);
+            fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, ilmax, Wl, tl, tlstride);
+            row_phases<T, SET>(ph, g, 2);
+#pragma unroll
+            for (int a = 0; a < L0; ++a)
+#pragma unroll
+                for (int i = 0; i < L; ++i) {
+                    const cx<T> f = Ft[PD.idx[i][a]] * T(PD.sg[i][a]);
+                    cfma(ptrac, U[a][i] * ph[i], f);
+                }
+        }
+    }
+    // cfac = exp(i R (ck - xk)) / prod_j (ck r_j), R = sum r_j = 2 reff (src/feffgpu/stage.py:336-337); the phase
+    // R Re(ck - xk) in FP64 reduced mod 2 pi
+    const double dkr = ed.ckmxk[2 * ee], dki = ed.ckmxk[2 * ee + 1];
+    double phs = g.R * dkr;
+    phs -= FM_TWO_PI * rint(phs / FM_TWO_PI);
+    const double damp = exp(-g.R * dki);
+    T sp, cp;
+    if constexpr (sizeof(T) == 4) {
+        sincosf((float)phs, &sp, &cp);
+    } else {
+        sincos(phs, &sp, &cp);
+    }
+    cx<T> num = mk<T>((T)damp * cp, (T)damp * sp);
+    cx<T> ckn = ck;
+    for (int j = 1; j < n; ++j) ckn = ckn * ck;
+    cx<T> den = ckn * (T)g.prod_r;
+    return cdiv(ptrac * num, den);
+}
+
+// Fine-grid epilogue for the path held by the wave: amp/phase of cchi (lanes = energies, already in LDS as
+// amp[] and ang[]), unwrap (FP64 scan), interpolation to q, chi = A sin(Phi + 2 q reff) exp(-2 sigma2 q^2).
+template <class T>
+__device__ void fine_grid(const fm_edge &ed, const int32_t *jq, const double *fq, const double *q, int nk, double reff_A,
+                          double sigma2, T *amp, double *ang, T *out) {
+    const int lane = lane_id();
+    const int E = ed.E;
+    // unwrap: pha[e] = ang[0] + sum_{i<=e} wrap(ang[i] - ang[i-1]) (src/feffgpu/stage.py:365-370)
+    const double a0 = ang[0];
+    double dl[FM_MAX_ECHUNK];
+#pragma unroll
+    for (int c = 0; c < FM_MAX_ECHUNK; ++c) {
+        const int e = c * FM_WAVE + lane;
+        double d = 0.0;
+        if (e < E && e > 0) {
+            d = ang[e] - ang[e - 1];
+            d -= FM_TWO_PI * rint(d / FM_TWO_PI);
+        }
+        dl[c] = d;
+    }
+    wave_sync();
+    double carry = 0.0;
+#pragma unroll
+    for (int c = 0; c < FM_MAX_ECHUNK; ++c) {
+        if (c * FM_WAVE >= E) break;
+        const int e = c * FM_WAVE + lane;
+        const double s = wave_scan_add(dl[c]) + carry;
+        carry = __shfl(s, FM_WAVE - 1, FM_WAVE);
+        if (e < E) ang[e] = a0 + s;
+    }
+    wave_sync();
+    for (int i = lane; i < nk; i += FM_WAVE) {
+        const int j = jq[i];
+        const double f = fq[i];
+        const T A = (T)((1.0 - f) * (double)amp[j] + f * (double)amp[j + 1]);
+        double ph = (1.0 - f) * ang[j] + f * ang[j + 1] + 2.0 * q[i] * reff_A;
+        ph -= FM_TWO_PI * rint(ph / FM_TWO_PI);
+        T v;
+        if constexpr (sizeof(T) == 4) v = A * sinf((float)ph);
+        else v = A * sin(ph);
+        if (sigma2 != 0.0) v *= (T)exp(-2.0 * sigma2 * q[i] * q[i]);
+        out[i] = v;
+    }
+}
+
+
+}  // namespace fm

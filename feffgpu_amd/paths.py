@@ -211,6 +211,22 @@ class PathPlan:
         return chi_seg, self.chi_path, self.reff
 
 
+    def forward_bucket(self, pos, b):
+        """Only bucket b of the plan (no segment sum): one kernel launch, for timing the kernel variants."""
+        pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
+        if self.max_atom >= pos64.shape[0]:
+            raise ValueError("pos has too few rows for the path list")
+        off = (N._i32 * 2)(int(self.buckets.offset[b]), int(self.buckets.offset[b + 1]))
+        kind = (N._i32 * 1)(int(self.buckets.kind[b]))
+        nlm = (N._i32 * 1)(int(self.buckets.nlegmax[b]))
+        ps = self._paths_struct(pos64)
+        N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
+                                     self.order_t.data_ptr(), 1, off, kind, nlm, 0, None, None, None,
+                                     self.chi_path.data_ptr(), self.reff.data_ptr(), N.stream_of(self.device)),
+                "fm_chi_paths")
+        return self.chi_path
+
+
 def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, nseg=None, edge=None, sigma2=None,
               cell=None, precision=32, device="cuda"):
     """One-shot fused evaluation: returns (chi_seg (nseg, NK), chi_path (P, NK), reff (P,))."""

@@ -74,6 +74,11 @@ typedef struct fm_grid {
     const double *q;             /* [nk] 1/A */
     const int32_t *jq;           /* [nedge][nk] lower phase-grid index, clipped to 0..E-2 */
     const double *fq;            /* [nedge][nk] weight of the upper point */
+    /* for the adjoints (may be NULL for the forward calls): q sorted ascending is also sorted by jq on every
+     * edge, so the points interpolating from energy e form one run of qorder */
+    const int32_t *qorder;       /* [nk] indices of q in ascending order */
+    const int32_t *qstart;       /* [nedge][qstride] first position in qorder whose jq >= e, e = 0..E_edge */
+    int32_t qstride;             /* row length of qstart (>= largest E + 1) */
 } fm_grid;
 
 /* GPUPathStage.cchi (src/feffgpu/stage.py:235-338): npath paths of nleg legs each; coords [npath][nleg][3] (A,
@@ -93,7 +98,8 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
 int fm_chi_fine(const fm_edge *edge, int32_t precision, int32_t npath, const void *cchi, const double *reff,
                 const fm_grid *grid, void *chi, void *stream);
 
-/* Adjoint of fm_chi_fine: gchi [npath][nk] (element type of chi) -> gcchi [npath][E] complex and greff [npath]. */
+/* Adjoint of fm_chi_fine: gchi [npath][nk] (element type of chi) -> gcchi [npath][E] complex and greff [npath]
+ * (grid.qorder/qstart required). */
 int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, const void *cchi, const double *reff,
                          const fm_grid *grid, const void *gchi, void *gcchi, double *greff, void *stream);
 
@@ -114,14 +120,18 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
                  const int32_t *bucket_kind, const int32_t *bucket_nlegmax, int32_t nseg, const int32_t *seg_ptr,
                  const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff, void *stream);
 
-/* Adjoint of fm_chi_paths: gseg [nseg][nk] = dL/dchi_seg (double), gpath [npath][nk] = dL/dchi_path (element type
- * by precision) or NULL. Writes gcoord [npath][nmax][3] (workspace, double) and grad_pos [N][3] = dL/dpos, summed
- * deterministically per atom over atom_ptr [N + 1] / atom_refs [*] (CSR of (path * nmax + slot) references, slot 0
- * meaning the absorber). */
+/* Adjoint of fm_chi_paths: the same path list, buckets and grid (grid.qorder/qstart required); gseg [nseg][nk] =
+ * dL/dchi_seg (double) with segment [npath] the row of each path, gpath [npath][nk] = dL/dchi_path (element type by
+ * precision) or NULL. Writes gcoord [npath][nmax][3] (double; dL/dcoords of each path's scatterers in slots
+ * 1..nleg-1 and minus their sum, the absorber's share, in slot 0) and grad_pos [natom][3] = dL/dpos, the sum over
+ * atom_ptr [natom + 1] / atom_refs (CSR of path * nmax + slot references, device), in list order. The forward is
+ * recomputed inside (nothing from the forward call is kept). */
 int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
-                          int32_t precision, int32_t iorder, const int32_t *order, const int32_t *segment,
-                          const double *gseg, const void *gpath, int32_t natom, const int32_t *atom_ptr,
-                          const int32_t *atom_refs, double *gcoord, double *grad_pos, void *stream);
+                          int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
+                          const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_nlegmax,
+                          const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
+                          const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
+                          void *stream);
 
 /* Message of the last failed call on this thread. */
 const char *fm_last_error(void);
