@@ -38,7 +38,25 @@ class FmPaths(ctypes.Structure):
 
 
 class FmGrid(ctypes.Structure):
-    _fields_ = [("nk", _i32), ("q", _vp), ("jq", _vp), ("fq", _vp)]
+    _fields_ = [("nk", _i32), ("q", _vp), ("jq", _vp), ("fq", _vp), ("qorder", _vp), ("qstart", _vp), ("qstride", _i32)]
+
+
+def grid_runs(q, jq_rows, E_list):
+    """qorder (q indices in ascending q) and qstart [nedge][stride]: for every edge, the first position in qorder
+    whose lower interpolation index is >= e, for e = 0..E (the runs the adjoint of the fine grid gathers)."""
+    jq_rows = [np.asarray(j) for j in jq_rows]
+    stride = max(E_list) + 1
+    order = np.argsort(np.asarray(q), kind="stable")
+    qstart = np.zeros((len(jq_rows), stride), np.int32)
+    for r, (j, E) in enumerate(zip(jq_rows, E_list)):
+        js = j[order]
+        if np.any(np.diff(js) < 0):
+            order = None
+            break
+        qstart[r] = np.searchsorted(js, np.arange(stride), side="left")
+    if order is None:
+        raise ValueError("the interpolation indices of the grid are not monotone in a common order")
+    return order.astype(np.int32), qstart, stride
 
 
 class NativeError(RuntimeError):
@@ -66,7 +84,8 @@ def lib():
                                    _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
                                    _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fm_chi_paths_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths),
-                                            ctypes.POINTER(FmGrid), _i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
+                                            ctypes.POINTER(FmGrid), _i32, _i32, _vp, _i32, ctypes.POINTER(_i32),
+                                            ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp, _vp, _i32, _vp, _vp,
                                             _vp, _vp, _vp]
         _lib = L
     return _lib

@@ -138,8 +138,12 @@ class PathPlan:
         self.q_t = torch.tensor(self.q, device=dev)
         self.jq_t = torch.tensor(np.stack(jq), device=dev, dtype=torch.int32)
         self.fq_t = torch.tensor(np.stack(fq), device=dev, dtype=torch.float64)
+        qorder, qstart, qstride = N.grid_runs(self.q, jq, [T.E for T in self.tables])
+        self.qorder_t = torch.tensor(qorder, device=dev)
+        self.qstart_t = torch.tensor(qstart, device=dev)
         self.grid = N.FmGrid()
         self.grid.nk, self.grid.q, self.grid.jq, self.grid.fq = nk, self.q_t.data_ptr(), self.jq_t.data_ptr(), self.fq_t.data_ptr()
+        self.grid.qorder, self.grid.qstart, self.grid.qstride = self.qorder_t.data_ptr(), self.qstart_t.data_ptr(), qstride
         # path arrays
         self.npath, self.nmax, self.nk = Pn, nmax, nk
         self.nleg_h, self.edge_h, self.segment_h, self.deg_h = nleg, edge, segment, deg
@@ -171,6 +175,20 @@ class PathPlan:
         np.add.at(seg_ptr, segment + 1, 1)
         self.seg_ptr_t = i32(np.cumsum(seg_ptr))
         self.seg_paths_t = i32(seg_order)
+        self.segment_t = i32(segment)
+        # per-atom references (path * nmax + slot; slot 0 = the absorber) for the gradient gather
+        slots = np.arange(nmax)[None, :]
+        live = slots < nleg[:, None]
+        who = np.concatenate([absorber[:, None], atoms], 1) if Pn else np.zeros((0, nmax), np.int64)
+        ref_atom = who[live]
+        ref_id = (np.arange(Pn)[:, None] * nmax + slots)[live]
+        o = np.lexsort((ref_id, ref_atom))
+        self.natom_ref = int(self.max_atom + 1)
+        ptr = np.zeros(self.natom_ref + 1, np.int64)
+        np.add.at(ptr, ref_atom + 1, 1)
+        self.atom_ptr_t = i32(np.cumsum(ptr))
+        self.atom_refs_t = i32(ref_id[o])
+        self.gcoord = torch.empty(Pn, nmax, 3, dtype=torch.float64, device=dev)
         self.chi_path = torch.empty(Pn, nk, dtype=self.rdtype, device=dev)
         self.reff = torch.empty(Pn, dtype=torch.float64, device=dev)
         self._h_off = (N._i32 * len(self.buckets.offset))(*self.buckets.offset.tolist())
@@ -211,6 +229,35 @@ class PathPlan:
         return chi_seg, self.chi_path, self.reff
 
 
+    def backward(self, pos, gseg=None, gpath=None):
+        """dL/dpos (N, 3) float64 given gseg = dL/dchi_seg (nseg, NK) and/or gpath = dL/dchi_path (P, NK), at the
+        positions pos (the forward is recomputed inside the kernel)."""
+        pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
+        if self.max_atom >= pos64.shape[0]:
+            raise ValueError("pos has too few rows for the path list")
+        Nat = pos64.shape[0]
+        grad = torch.zeros(Nat, 3, dtype=torch.float64, device=self.device)
+        if not self.npath or (gseg is None and gpath is None):
+            return grad
+        gs = None if gseg is None else torch.as_tensor(gseg).to(device=self.device, dtype=torch.float64).contiguous()
+        if gs is not None and tuple(gs.shape) != (self.nseg, self.nk):
+            raise ValueError("gseg must be (nseg, NK)")
+        gp = None if gpath is None else torch.as_tensor(gpath).to(device=self.device, dtype=self.rdtype).contiguous()
+        if gp is not None and tuple(gp.shape) != (self.npath, self.nk):
+            raise ValueError("gpath must be (P, NK)")
+        ps = self._paths_struct(pos64)
+        N.check(N.lib().fm_chi_paths_backward(self.edge_array, len(self.edges), ps, self.grid, self.precision,
+                                              self.iorder, self.order_t.data_ptr(), len(self.buckets.kind), self._h_off,
+                                              self._h_kind, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
+                                              N.ptr(gp), self.natom_ref, self.atom_ptr_t.data_ptr(),
+                                              self.atom_refs_t.data_ptr(), self.gcoord.data_ptr(), grad.data_ptr(),
+                                              N.stream_of(self.device)), "fm_chi_paths_backward")
+        return grad
+
+    def __call__(self, pos):
+        """Differentiable evaluation: (chi_seg (nseg, NK) float64, chi_path (P, NK)) with gradients to pos."""
+        return _ChiPathsFn.apply(torch.as_tensor(pos), self)
+
     def forward_bucket(self, pos, b):
         """Only bucket b of the plan (no segment sum): one kernel launch, for timing the kernel variants."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
@@ -225,6 +272,21 @@ class PathPlan:
                                      self.chi_path.data_ptr(), self.reff.data_ptr(), N.stream_of(self.device)),
                 "fm_chi_paths")
         return self.chi_path
+
+
+class _ChiPathsFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, plan):
+        seg, path, _ = plan.forward(pos)
+        ctx.plan = plan
+        ctx.save_for_backward(pos)
+        return seg, path.clone()
+
+    @staticmethod
+    def backward(ctx, gseg, gpath):
+        (pos,) = ctx.saved_tensors
+        g = ctx.plan.backward(pos, gseg, gpath)
+        return g.to(pos.dtype), None
 
 
 def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, nseg=None, edge=None, sigma2=None,

@@ -161,12 +161,17 @@ class GPUPathStage:
         g = self._grids.get(key)
         if g is None:
             j, f = interpolation(self.P.xk[: self.E] / BOHR, qa)
+            qorder = np.argsort(qa, kind="stable").astype(np.int32)
+            qstart = np.searchsorted(j[qorder], np.arange(self.E + 1), side="left").astype(np.int32)
             qt = torch.tensor(qa, device=self.device)
             jt = torch.tensor(j, device=self.device, dtype=torch.int32)
             ft = torch.tensor(f, device=self.device, dtype=torch.float64)
+            ot = torch.tensor(qorder, device=self.device)
+            st = torch.tensor(qstart, device=self.device)
             s = N.FmGrid()
             s.nk, s.q, s.jq, s.fq = len(qa), qt.data_ptr(), jt.data_ptr(), ft.data_ptr()
-            g = self._grids[key] = (s, qt, jt, ft)
+            s.qorder, s.qstart, s.qstride = ot.data_ptr(), st.data_ptr(), self.E + 1
+            g = self._grids[key] = (s, qt, jt, ft, ot, st)
         return g
 
     def chi_fine(self, cchi, reff, q, S=None, convention="feff"):
