@@ -76,7 +76,10 @@ def lib():
         L.fm_last_error.restype = ctypes.c_char_p
         L.fm_version.restype = ctypes.c_char_p
         L.fm_cchi.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp]
-        L.fm_cchi_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.fm_cchi_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                       ctypes.c_size_t, _vp]
+        L.fm_backward_workspace_bytes.restype = ctypes.c_size_t
+        L.fm_backward_workspace_bytes.argtypes = [_i32, _i32, _i32]
         L.fm_chi_fine.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _vp, _vp, ctypes.POINTER(FmGrid), _vp, _vp]
         L.fm_chi_fine_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _vp, _vp, ctypes.POINTER(FmGrid), _vp,
                                            _vp, _vp, _vp]
@@ -86,7 +89,7 @@ def lib():
         L.fm_chi_paths_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths),
                                             ctypes.POINTER(FmGrid), _i32, _i32, _vp, _i32, ctypes.POINTER(_i32),
                                             ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp, _vp, _i32, _vp, _vp,
-                                            _vp, _vp, _vp]
+                                            _vp, _vp, _vp, ctypes.c_size_t, _vp]
         _lib = L
     return _lib
 
@@ -103,6 +106,14 @@ def check(rc: int, what: str):
 
 def ptr(t) -> int | None:
     return None if t is None else t.data_ptr()
+
+
+def backward_workspace(precision: int, nlegmax: int, npath: int, device, max_slots: int = 4096):
+    """A device buffer for the adjoints' per-path prefix products: enough wave slots for min(npath, max_slots)
+    resident paths (rounded up to the workgroup size)."""
+    slots = max(4, min(int(max_slots), -(-max(int(npath), 1) // 4) * 4))
+    nbytes = lib().fm_backward_workspace_bytes(int(precision), int(nlegmax), slots)
+    return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
 
 
 def stream_of(device) -> int:

@@ -43,7 +43,9 @@ struct BArgs {
     double *gcoord;             // [npath][nslot][3]
     int nslot;
     int ltab, emax, nk, nlegmax, wpb;
-    size_t wave_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_gq, off_pre, off_gacc, off_gpar, off_tmp;
+    size_t wave_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_gq, off_gacc, off_gpar, off_tmp;
+    unsigned char *work;        // per wave slot: the prefix products (global memory, cache-resident)
+    size_t work_per_wave, work_bytes;
 };
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }
@@ -64,7 +66,7 @@ template <class T, int SET> __host__ void layout(BArgs &a) {
     a.off_cb = o; o += align16(sizeof(cx<T>) * a.emax);
     a.off_gq = o; o += align16(sizeof(T) * 2 * (size_t)a.nk);
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
-    a.off_pre = o; o += align16(sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE);
+    a.work_per_wave = (sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE + 255) & ~size_t(255);
     a.off_gacc = o; o += align16(sizeof(T) * 3 * (size_t)a.nlegmax * FM_WAVE);
     a.off_gpar = o; o += align16(sizeof(double) * (3 * (size_t)a.nlegmax + 1));
     a.off_tmp = o; o += align16(sizeof(double) * 3 * FM_MAX_LEGS);
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) __attribute__((amdgpu_waves_per_e
     cx<T> *cb = reinterpret_cast<cx<T> *>(base + a.off_cb);
     T *gA = reinterpret_cast<T *>(base + a.off_gq);
     T *gP = gA + a.nk;
-    PreStore<T, SET> pre{reinterpret_cast<cx<T> *>(base + a.off_pre)};
+    PreStore<T, SET> pre{reinterpret_cast<cx<T> *>(a.work + (size_t)(blockIdx.x * a.wpb + wid) * a.work_per_wave)};
     GAcc<T> acc{reinterpret_cast<T *>(base + a.off_gacc), a.nlegmax};
     double *gpar = reinterpret_cast<double *>(base + a.off_gpar);
     double *tmp = reinterpret_cast<double *>(base + a.off_tmp);
@@ -785,9 +787,12 @@ template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st)
         hipError_t e = hipFuncSetAttribute((const void *)k_backward<T, SET>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const int blocks = (a.npath_bucket + wpb - 1) / wpb;
+    const long slots = (long)(a.work_bytes / a.work_per_wave);
+    if (slots < wpb) return hipErrorInvalidValue;
+    long blocks = (a.npath_bucket + wpb - 1) / wpb;
+    if (blocks > slots / wpb) blocks = slots / wpb;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_backward<T, SET>), dim3(blocks), dim3(FM_WAVE * wpb), shm, st, a);
+    hipLaunchKernelGGL((k_backward<T, SET>), dim3((unsigned)blocks), dim3(FM_WAVE * wpb), shm, st, a);
     return hipGetLastError();
 }
 
@@ -825,17 +830,27 @@ int set_of(const fm_edge &ed, int nleg) {
 
 extern "C" {
 
+size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t wave_slots) {
+    if (nlegmax < 2 || nlegmax > FM_MAX_LEGS || wave_slots < 1) return 0;
+    const size_t es = precision == 32 ? 8 : 16;
+    const size_t nslots = nlegmax > 2 ? nlegmax - 1 : 1;
+    const size_t per = (es * nslots * 36 * FM_WAVE + 255) & ~size_t(255);
+    return per * (size_t)wave_slots;
+}
+
 int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg,
                      const double *coords, const int64_t *ipots, const void *gcchi, const double *greff, double *gcoords,
-                     void *stream) {
+                     void *workspace, size_t workspace_bytes, void *stream) {
     int rc = check_edge_b(edge);
     if (rc) return rc;
     if (iorder != 2) return set_err(FM_EUNSUPPORTED, "only iorder = 2 is implemented on the GPU");
     if (nleg < 2 || nleg > FM_MAX_LEGS) return set_err(FM_EINVAL, "nleg out of range 2..9");
     if (npath < 0) return set_err(FM_EINVAL, "npath < 0");
     if (npath == 0) return FM_OK;
-    if (!coords || !ipots || !gcchi || !gcoords) return set_err(FM_EINVAL, "NULL buffer");
+    if (!coords || !ipots || !gcchi || !gcoords || !workspace) return set_err(FM_EINVAL, "NULL buffer");
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
+    if (workspace_bytes < fm_backward_workspace_bytes(precision, nleg, WPB_MAX))
+        return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nleg, 4)");
     const int set = set_of(*edge, nleg);
     if (set < 0) return set_err(FM_EUNSUPPORTED, "two-leg paths of L and M edges are not implemented on the GPU");
     BArgs a;
@@ -855,6 +870,8 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     a.emax = edge->E;
     a.nk = 0;
     a.nlegmax = nleg;
+    a.work = (unsigned char *)workspace;
+    a.work_bytes = workspace_bytes;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi_backward launch");
@@ -892,7 +909,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_nlegmax,
                           const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          void *stream) {
+                          void *workspace, size_t workspace_bytes, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     int emax = 0, ltab = 0;
     for (int i = 0; i < nedge; ++i) {
@@ -908,6 +925,8 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     if (paths->nmax < 2 || paths->nmax > FM_MAX_LEGS) return set_err(FM_EINVAL, "nmax out of range 2..9");
     if (grid->qstride < emax + 1) return set_err(FM_EINVAL, "grid.qstride < E + 1");
+    if (!workspace || workspace_bytes < fm_backward_workspace_bytes(precision, paths->nmax, WPB_MAX))
+        return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nmax, 4)");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {
         const int cnt = bucket_offset[b + 1] - bucket_offset[b];
@@ -934,6 +953,8 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         a.nk = grid->nk;
         int nlm = bucket_nlegmax ? bucket_nlegmax[b] : paths->nmax;
         a.nlegmax = (nlm < 2 || nlm > paths->nmax) ? paths->nmax : nlm;
+        a.work = (unsigned char *)workspace;
+        a.work_bytes = workspace_bytes;
         hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths_backward launch");
     }

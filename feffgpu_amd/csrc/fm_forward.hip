@@ -125,19 +125,32 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine(fm_edge ed, int n
     }
 }
 
-// chi_seg[s][k] = sum over the CSR list of segment s of deg[p] chi_path[p][k], in list order (deterministic).
+// chi_seg[s][k] = sum over the CSR list of segment s of deg[p] chi_path[p][k]: a workgroup per (64 k-points,
+// segment); wave w sums the list entries w, w + 8, ... in order, then the 8 partial sums are added in wave order.
+// Deterministic for a given list and grid.
+constexpr int SEG_WAVES = 8;
 template <class T>
-__global__ void k_segment_sum(int nseg, int nk, const int32_t *seg_ptr, const int32_t *seg_paths, const double *deg,
-                              const T *chi_path, double *chi_seg) {
+__global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, int nk, const int32_t *seg_ptr,
+                                                                     const int32_t *seg_paths, const double *deg,
+                                                                     const T *chi_path, double *chi_seg) {
+    __shared__ double part[SEG_WAVES][FM_WAVE];
     const int s = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg || k >= nk) return;
+    const int w = threadIdx.x / FM_WAVE, lane = threadIdx.x % FM_WAVE;
+    const int k = blockIdx.x * FM_WAVE + lane;
     double acc = 0.0;
-    for (int t = seg_ptr[s]; t < seg_ptr[s + 1]; ++t) {
-        const int p = seg_paths[t];
-        acc += deg[p] * (double)chi_path[(size_t)p * nk + k];
+    if (k < nk) {
+        for (int t = seg_ptr[s] + w; t < seg_ptr[s + 1]; t += SEG_WAVES) {
+            const int p = seg_paths[t];
+            acc += deg[p] * (double)chi_path[(size_t)p * nk + k];
+        }
     }
-    chi_seg[(size_t)s * nk + k] = acc;
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && k < nk) {
+        double tot = 0.0;
+        for (int i = 0; i < SEG_WAVES; ++i) tot += part[i][lane];
+        chi_seg[(size_t)s * nk + k] = tot;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ launch helpers
@@ -294,13 +307,13 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
     }
     if (chi_seg && nseg > 0) {
         if (!seg_ptr || !seg_paths) return set_err(FM_EINVAL, "NULL segment lists");
-        dim3 grd((grid->nk + 127) / 128, nseg);
+        dim3 grd((grid->nk + FM_WAVE - 1) / FM_WAVE, nseg);
         if (precision == 32)
-            hipLaunchKernelGGL(k_segment_sum<float>, grd, dim3(128), 0, st, nseg, grid->nk, seg_ptr, seg_paths, paths->deg,
-                               (const float *)chi_path, chi_seg);
+            hipLaunchKernelGGL(k_segment_sum<float>, grd, dim3(FM_WAVE * SEG_WAVES), 0, st, nseg, grid->nk, seg_ptr, seg_paths,
+                               paths->deg, (const float *)chi_path, chi_seg);
         else
-            hipLaunchKernelGGL(k_segment_sum<double>, grd, dim3(128), 0, st, nseg, grid->nk, seg_ptr, seg_paths, paths->deg,
-                               (const double *)chi_path, chi_seg);
+            hipLaunchKernelGGL(k_segment_sum<double>, grd, dim3(FM_WAVE * SEG_WAVES), 0, st, nseg, grid->nk, seg_ptr, seg_paths,
+                               paths->deg, (const double *)chi_path, chi_seg);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_err(e, "segment sum launch");
     }

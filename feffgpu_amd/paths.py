@@ -189,6 +189,7 @@ class PathPlan:
         self.atom_ptr_t = i32(np.cumsum(ptr))
         self.atom_refs_t = i32(ref_id[o])
         self.gcoord = torch.empty(Pn, nmax, 3, dtype=torch.float64, device=dev)
+        self.workspace = N.backward_workspace(self.precision, nmax, Pn, dev) if Pn else None
         self.chi_path = torch.empty(Pn, nk, dtype=self.rdtype, device=dev)
         self.reff = torch.empty(Pn, dtype=torch.float64, device=dev)
         self._h_off = (N._i32 * len(self.buckets.offset))(*self.buckets.offset.tolist())
@@ -251,6 +252,7 @@ class PathPlan:
                                               self._h_kind, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
                                               N.ptr(gp), self.natom_ref, self.atom_ptr_t.data_ptr(),
                                               self.atom_refs_t.data_ptr(), self.gcoord.data_ptr(), grad.data_ptr(),
+                                              self.workspace.data_ptr(), self.workspace.numel(),
                                               N.stream_of(self.device)), "fm_chi_paths_backward")
         return grad
 

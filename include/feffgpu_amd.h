@@ -91,7 +91,11 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
  * [npath] dL/dreff or NULL. Writes gcoords [npath][nleg][3] = dL/dcoords. */
 int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg,
                      const double *coords, const int64_t *ipots, const void *gcchi, const double *greff, double *gcoords,
-                     void *stream);
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* Bytes of the workspace the adjoints need for `wave_slots` concurrently resident paths (each keeps its prefix
+ * products there; the launch uses as many slots as the workspace holds, at least 4). No device call. */
+size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t wave_slots);
 
 /* GPUPathStage.chi_fine with the linear ("feff") convention (src/feffgpu/stage.py:365-388): chi [npath][nk] =
  * A(q) sin(Phi(q) + 2 q reff). cchi as written by fm_cchi; jq/fq of grid are read for edge 0. */
@@ -131,7 +135,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_nlegmax,
                           const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          void *stream);
+                          void *workspace, size_t workspace_bytes, void *stream);
 
 /* Message of the last failed call on this thread. */
 const char *fm_last_error(void);

@@ -14,7 +14,7 @@ HEADER = ROOT / "include" / "feffgpu_amd.h"
 def declared_functions():
     txt = HEADER.read_text()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(fm_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*(fm_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -24,7 +24,8 @@ def test_library_exports_every_declared_symbol():
         build()
     lib = ctypes.CDLL(str(N.LIB_PATH))
     names = declared_functions()
-    assert "fm_chi_paths" in names and "fm_cchi" in names and len(names) >= 8
+    assert "fm_chi_paths" in names and "fm_cchi" in names and "fm_backward_workspace_bytes" in names
+    assert len(names) >= 9
     for name in names:
         assert hasattr(lib, name), name
     lib.fm_version.restype = ctypes.c_char_p
