@@ -28,7 +28,7 @@ class FmEdge(ctypes.Structure):
     _fields_ = [("E", _i32), ("L1", _i32), ("nph1", _i32), ("ilinit", _i32), ("nchan", _i32),
                 ("chan_il", _i32 * FM_MAX_CHAN), ("lmax1", _i32 * FM_MAX_POT),
                 ("lmax", _vp), ("ck", _vp), ("ckmxk", _vp), ("kf", _vp), ("tl32", _vp), ("tl64", _vp),
-                ("chw32", _vp), ("chw64", _vp), ("rotc", _vp), ("rotn", _vp)]
+                ("chw32", _vp), ("chw64", _vp), ("rotc", _vp), ("rotn", _vp), ("gpaths", _i32), ("gstart", _i32 * 5)]
 
 
 class FmPaths(ctypes.Structure):
@@ -85,11 +85,11 @@ def lib():
                                            _vp, _vp, _vp]
         L.fm_chi_paths.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid), _i32,
                                    _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
-                                   _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+                                   ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fm_chi_paths_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths),
                                             ctypes.POINTER(FmGrid), _i32, _i32, _vp, _i32, ctypes.POINTER(_i32),
-                                            ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp, _vp, _i32, _vp, _vp,
-                                            _vp, _vp, _vp, ctypes.c_size_t, _vp]
+                                            ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp,
+                                            _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
         _lib = L
     return _lib
 
@@ -157,4 +157,7 @@ class DeviceEdge:
         s.lmax, s.ck, s.ckmxk, s.kf = (t.data_ptr() for t in (self.lmax, self.ck, self.ckmxk, self.kf))
         s.tl32, s.tl64, s.chw32, s.chw64 = (t.data_ptr() for t in (self.tl32, self.tl64, self.chw32, self.chw64))
         s.rotc, s.rotn = self.rotc.data_ptr(), self.rotn.data_ptr()
+        s.gpaths = int(T.gpaths)
+        for i, v in enumerate(T.gstart):
+            s.gstart[i] = int(v)
         self.struct = s

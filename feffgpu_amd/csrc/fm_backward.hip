@@ -906,8 +906,8 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
 
 int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                           int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
-                          const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_nlegmax,
-                          const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
+                          const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
+                          const int32_t *bucket_nlegmax, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
                           void *workspace, size_t workspace_bytes, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
@@ -933,6 +933,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         if (cnt <= 0) continue;
         const int set = bucket_kind[b];
         if (set < 0 || set > 2) return set_err(FM_EUNSUPPORTED, "bucket kind not implemented on the GPU");
+        if (bucket_edge && (bucket_edge[b] < 0 || bucket_edge[b] >= nedge)) return set_err(FM_EINVAL, "bucket edge out of range");
         BArgs a;
         memset(&a, 0, sizeof a);
         for (int i = 0; i < nedge; ++i) a.ed[i] = edges[i];

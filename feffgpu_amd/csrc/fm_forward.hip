@@ -42,8 +42,8 @@ struct KArgs {
     double *reff_out;
     int ltab;             // L1 + 1 rows per rotation table
     int emax;             // largest E over the edges
-    int wpb;              // wavefronts per workgroup
-    size_t wave_bytes;
+    int edge_id;          // the edge of every path of this launch
+    size_t path_bytes;    // LDS per path of a workgroup
 };
 
 int set_err(int code, const char *msg) {
@@ -56,50 +56,81 @@ int hip_err(hipError_t e, const char *where) {
 }
 
 
+// A workgroup holds G = ed.gpaths paths and 4 wavefronts. Setup: wave j < G loads path j (geometry, rotation tables)
+// into its LDS area. Energies: wave w takes the energy band [gstart[w], gstart[w+1]) of all G paths, lane
+// (path j, energy i) = j * (64 / G) + i: the lanes of a wave share similar partial-wave counts. Epilogue: wave j
+// interpolates path j to the output grid.
 template <class T, int SET>
-__global__ __launch_bounds__(FM_WAVE *WPB_MAX) __attribute__((amdgpu_waves_per_eu(FM_WPE(T)))) void k_forward(KArgs a) {
+__global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_WPE(T)))) void k_forward(KArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / FM_WAVE;
     const int lane = lane_id();
-    unsigned char *base = smem + (size_t)wid * a.wave_bytes;
-    PathLDS &g = *reinterpret_cast<PathLDS *>(base);
-    T *W = reinterpret_cast<T *>(base + align16(sizeof(PathLDS)));
-    for (int slot = blockIdx.x * a.wpb + wid; slot < a.npath_bucket; slot += gridDim.x * a.wpb) {
-        const int p = a.order ? a.order[a.first + slot] : a.first + slot;
-        int ei;
-        const int n = load_path(a.src, p, g, ei);
-        const fm_edge &ed = a.ed[ei];
-        const int ltab = ed.L1 + 1;
-        build_rotation<T, SET>(ed, g, n, ltab, W);
-        T *amp = reinterpret_cast<T *>(base + a.wave_bytes - align16(sizeof(double) * (size_t)a.emax) -
-                                       align16(sizeof(T) * (size_t)a.emax));
-        double *ang = reinterpret_cast<double *>(base + a.wave_bytes - align16(sizeof(double) * (size_t)a.emax));
-        for (int e0 = 0; e0 < ed.E; e0 += FM_WAVE) {
-            const int e = e0 + lane;
-            cx<T> c = path_cchi<T, SET>(ed, g, W, ltab, e, n);
-            if (e < ed.E) {
+    const fm_edge &ed = a.ed[a.edge_id];
+    const int G = ed.gpaths, cap = FM_WAVE / G;
+    const int ltab = ed.L1 + 1, E = ed.E;
+    const size_t off_W = align16(sizeof(PathLDS));
+    const int nl = a.src.mode == 0 ? a.src.nleg : a.src.P.nmax;     // rotation tables sized for the longest path
+    const size_t off_amp = off_W + align16(sizeof(T) * (size_t)nl * ltab * FM_NREP);
+    const size_t off_ang = off_amp + align16(sizeof(T) * (size_t)E);
+    __shared__ int pidx[4];
+    __shared__ int pval[4];
+    for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
+        // ---- setup: one wave per path
+        if (wid < G) {
+            const int slot = blk * G + wid;
+            const bool valid = slot < a.npath_bucket;
+            const int s0 = valid ? slot : blk * G;
+            const int p = a.order ? a.order[a.first + s0] : a.first + s0;
+            unsigned char *base = smem + (size_t)wid * a.path_bytes;
+            PathLDS &g = *reinterpret_cast<PathLDS *>(base);
+            int ei;
+            const int n = load_path(a.src, p, g, ei);
+            build_rotation<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + off_W));
+            if (lane == 0) { pidx[wid] = p; pval[wid] = valid ? 1 : 0; }
+        }
+        __syncthreads();
+        // ---- energies: band wid of every path
+        {
+            const int e0 = ed.gstart[wid], nb = ed.gstart[wid + 1] - e0;
+            const int j = lane / cap, i = lane % cap;
+            const bool active = nb > 0 && i < nb && j < G;
+            const int jj = j < G ? j : G - 1;
+            const int e = e0 + (i < nb ? i : (nb > 0 ? nb - 1 : 0));
+            unsigned char *base = smem + (size_t)jj * a.path_bytes;
+            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+            const T *W = reinterpret_cast<const T *>(base + off_W);
+            const int ee = e < E ? e : E - 1;
+            const cx<T> c = path_cchi<T, SET>(ed, g, W, ltab, ee, g.n);
+            if (active && pval[jj]) {
                 if (a.src.mode == 0) {
-                    T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)p * ed.E + e) * 2;
+                    T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)pidx[jj] * E + e) * 2;
                     o[0] = c.r;
                     o[1] = c.i;
                 } else {
+                    T *amp = reinterpret_cast<T *>(base + off_amp);
+                    double *ang = reinterpret_cast<double *>(base + off_ang);
                     amp[e] = (T)sqrt((double)c.r * c.r + (double)c.i * c.i + 1e-30);
                     ang[e] = (double)atan2(c.i, c.r);
                 }
             }
         }
-        const double reff_A = g.reff * FM_BOHR;
-        if (a.src.mode == 0) {
+        __syncthreads();
+        // ---- epilogue: one wave per path
+        if (wid < G && pval[wid]) {
+            unsigned char *base = smem + (size_t)wid * a.path_bytes;
+            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+            const int p = pidx[wid];
+            const double reff_A = g.reff * FM_BOHR;
             if (lane == 0 && a.reff_out) a.reff_out[p] = reff_A;
-        } else {
-            wave_sync();
-            if (lane == 0 && a.reff_out) a.reff_out[p] = reff_A;
-            const int nk = a.G.nk;
-            T *out = reinterpret_cast<T *>(a.chi_path) + (size_t)p * nk;
-            fine_grid<T>(ed, a.G.jq + (size_t)ei * nk, a.G.fq + (size_t)ei * nk, a.G.q, nk, reff_A,
-                         a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, amp, ang, out);
+            if (a.src.mode == 1) {
+                const int nk = a.G.nk;
+                T *out = reinterpret_cast<T *>(a.chi_path) + (size_t)p * nk;
+                fine_grid<T>(ed, a.G.jq + (size_t)a.edge_id * nk, a.G.fq + (size_t)a.edge_id * nk, a.G.q, nk, reff_A,
+                             a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, reinterpret_cast<T *>(base + off_amp),
+                             reinterpret_cast<double *>(base + off_ang), out);
+            }
         }
-        wave_sync();
+        __syncthreads();
     }
 }
 
@@ -160,26 +191,22 @@ int set_for(const fm_edge &ed, int nleg) {
 }
 
 template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st, int nlegmax) {
-    int ltab = 0, emax = 0;
-    for (int i = 0; i < a.nedge; ++i) {
-        ltab = ltab > a.ed[i].L1 + 1 ? ltab : a.ed[i].L1 + 1;
-        emax = emax > a.ed[i].E ? emax : a.ed[i].E;
-    }
-    a.ltab = ltab;
-    a.emax = emax;
-    a.wave_bytes = wave_lds_bytes<T>(nlegmax, ltab, emax);
-    if (a.wave_bytes > 160 * 1024) return hipErrorInvalidValue;
-    int wpb = (int)((64 * 1024) / a.wave_bytes);
-    wpb = wpb < 1 ? 1 : (wpb > WPB_MAX ? WPB_MAX : wpb);
-    a.wpb = wpb;
-    const size_t shm = a.wave_bytes * wpb;
+    const fm_edge &ed = a.ed[a.edge_id];
+    a.ltab = ed.L1 + 1;
+    a.emax = ed.E;
+    const int nl = a.src.mode == 0 ? a.src.nleg : a.src.P.nmax;
+    (void)nlegmax;
+    a.path_bytes = align16(sizeof(PathLDS)) + align16(sizeof(T) * (size_t)nl * a.ltab * FM_NREP) + align16(sizeof(T) * ed.E) +
+                   align16(sizeof(double) * ed.E);
+    const size_t shm = a.path_bytes * ed.gpaths;
+    if (shm > 160 * 1024) return hipErrorInvalidValue;
     if (shm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_forward<T, SET>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const int blocks = (a.npath_bucket + wpb - 1) / wpb;
+    const int blocks = (a.npath_bucket + ed.gpaths - 1) / ed.gpaths;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_forward<T, SET>), dim3(blocks), dim3(FM_WAVE * wpb), shm, st, a);
+    hipLaunchKernelGGL((k_forward<T, SET>), dim3(blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
@@ -205,6 +232,11 @@ int check_edge(const fm_edge *e) {
         if (e->lmax1[i] < 1 || e->lmax1[i] > e->L1) return set_err(FM_EINVAL, "edge: lmax1 out of range");
     if (e->ilinit < 1 || e->ilinit > 3) return set_err(FM_EUNSUPPORTED, "edge: only K, L and M edges (ilinit 1..3)");
     if (!e->lmax || !e->ck || !e->ckmxk || !e->rotc || !e->rotn) return set_err(FM_EINVAL, "edge: NULL table");
+    if (e->gpaths != 1 && e->gpaths != 2 && e->gpaths != 4) return set_err(FM_EINVAL, "edge: gpaths must be 1, 2 or 4");
+    if (e->gstart[0] != 0 || e->gstart[4] != e->E) return set_err(FM_EINVAL, "edge: energy bands must cover 0..E");
+    for (int w = 0; w < 4; ++w)
+        if (e->gstart[w + 1] < e->gstart[w] || e->gstart[w + 1] - e->gstart[w] > FM_WAVE / e->gpaths)
+            return set_err(FM_EINVAL, "edge: an energy band is longer than 64 / gpaths");
     return FM_OK;
 }
 
@@ -240,6 +272,7 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
     a.src.ipots64 = ipots;
     a.cchi_out = cchi;
     a.reff_out = reff;
+    a.edge_id = 0;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = precision == 32 ? dispatch<float>(a, set, st, nleg) : dispatch<double>(a, set, st, nleg);
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi launch");
@@ -269,8 +302,8 @@ int fm_chi_fine(const fm_edge *edge, int32_t precision, int32_t npath, const voi
 
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                          int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
-                         const int32_t *bucket_offset, const int32_t *bucket_set, const int32_t *bucket_nlegmax,
-                         int32_t nseg, const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg,
+                         const int32_t *bucket_offset, const int32_t *bucket_set, const int32_t *bucket_edge,
+                         const int32_t *bucket_nlegmax, int32_t nseg, const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg,
                          void *chi_path, double *reff, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     for (int i = 0; i < nedge; ++i) {
@@ -300,6 +333,8 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         a.G = *grid;
         a.chi_path = chi_path;
         a.reff_out = reff;
+        a.edge_id = bucket_edge ? bucket_edge[b] : 0;
+        if (a.edge_id < 0 || a.edge_id >= nedge) return set_err(FM_EINVAL, "bucket edge out of range");
         int nlm = bucket_nlegmax ? bucket_nlegmax[b] : paths->nmax;
         if (nlm < 2 || nlm > paths->nmax) nlm = paths->nmax;
         hipError_t e = precision == 32 ? dispatch<float>(a, set, st, nlm) : dispatch<double>(a, set, st, nlm);

@@ -46,12 +46,14 @@ class Buckets:
     order: np.ndarray          # permutation of the paths
     offset: np.ndarray         # (nb + 1,)
     kind: np.ndarray           # (nb,)
+    edge: np.ndarray           # (nb,)
     nlegmax: np.ndarray        # (nb,)
 
 
 def bucket_paths(nleg: np.ndarray, edge: np.ndarray, ilinit: list) -> Buckets:
-    """Group the paths by kernel variant: (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (K/L1 edge, 2 legs); inside
-    a bucket the paths are ordered by decreasing number of legs (longest first, so the tail of a launch is short)."""
+    """Group the paths by kernel variant -- (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (K/L1 edge, 2 legs) -- and by
+    edge; inside a bucket the paths are ordered by decreasing number of legs, so a workgroup's paths share their
+    number of legs and the tail of a launch is short."""
     nleg = np.asarray(nleg)
     edge = np.asarray(edge)
     il = np.asarray(ilinit)[edge]
@@ -59,16 +61,17 @@ def bucket_paths(nleg: np.ndarray, edge: np.ndarray, ilinit: list) -> Buckets:
                     np.where(il <= 1, N.FM_BUCKET_K_MS, N.FM_BUCKET_LM_MS))
     if (kind < 0).any():
         raise NotImplementedError("two-leg paths of L and M edges are not implemented on the GPU")
-    order = np.lexsort((np.arange(len(nleg)), -nleg, kind)).astype(np.int32)
-    ks = kind[order]
-    kinds = [k for k in (N.FM_BUCKET_K_MS, N.FM_BUCKET_LM_MS, N.FM_BUCKET_K_SS) if (ks == k).any()]
-    off = [0]
-    nlm = []
-    for k in kinds:
-        sel = ks == k
+    order = np.lexsort((np.arange(len(nleg)), -nleg, edge, kind)).astype(np.int32)
+    key = (kind * 16 + edge)[order]
+    off, kinds, edges, nlm = [0], [], [], []
+    for kv in np.unique(key):
+        sel = key == kv
         off.append(off[-1] + int(sel.sum()))
+        kinds.append(int(kv) // 16)
+        edges.append(int(kv) % 16)
         nlm.append(int(nleg[order][sel].max()))
-    return Buckets(order, np.array(off, np.int32), np.array(kinds, np.int32), np.array(nlm, np.int32))
+    return Buckets(order, np.array(off, np.int32), np.array(kinds, np.int32), np.array(edges, np.int32),
+                   np.array(nlm, np.int32))
 
 
 class PathPlan:
@@ -194,6 +197,7 @@ class PathPlan:
         self.reff = torch.empty(Pn, dtype=torch.float64, device=dev)
         self._h_off = (N._i32 * len(self.buckets.offset))(*self.buckets.offset.tolist())
         self._h_kind = (N._i32 * max(1, len(self.buckets.kind)))(*self.buckets.kind.tolist())
+        self._h_edge = (N._i32 * max(1, len(self.buckets.edge)))(*self.buckets.edge.tolist())
         self._h_nlm = (N._i32 * max(1, len(self.buckets.nlegmax)))(*self.buckets.nlegmax.tolist())
 
     def _paths_struct(self, pos64):
@@ -222,7 +226,7 @@ class PathPlan:
             ps = self._paths_struct(pos64)
             N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
                                          self.order_t.data_ptr(), len(self.buckets.kind), self._h_off, self._h_kind,
-                                         self._h_nlm, self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(),
+                                         self._h_edge, self._h_nlm, self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(),
                                          N.ptr(chi_seg), self.chi_path.data_ptr(), self.reff.data_ptr(),
                                          N.stream_of(self.device)), "fm_chi_paths")
         elif want_seg:
@@ -249,7 +253,7 @@ class PathPlan:
         ps = self._paths_struct(pos64)
         N.check(N.lib().fm_chi_paths_backward(self.edge_array, len(self.edges), ps, self.grid, self.precision,
                                               self.iorder, self.order_t.data_ptr(), len(self.buckets.kind), self._h_off,
-                                              self._h_kind, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
+                                              self._h_kind, self._h_edge, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
                                               N.ptr(gp), self.natom_ref, self.atom_ptr_t.data_ptr(),
                                               self.atom_refs_t.data_ptr(), self.gcoord.data_ptr(), grad.data_ptr(),
                                               self.workspace.data_ptr(), self.workspace.numel(),
@@ -267,10 +271,11 @@ class PathPlan:
             raise ValueError("pos has too few rows for the path list")
         off = (N._i32 * 2)(int(self.buckets.offset[b]), int(self.buckets.offset[b + 1]))
         kind = (N._i32 * 1)(int(self.buckets.kind[b]))
+        edge = (N._i32 * 1)(int(self.buckets.edge[b]))
         nlm = (N._i32 * 1)(int(self.buckets.nlegmax[b]))
         ps = self._paths_struct(pos64)
         N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
-                                     self.order_t.data_ptr(), 1, off, kind, nlm, 0, None, None, None,
+                                     self.order_t.data_ptr(), 1, off, kind, edge, nlm, 0, None, None, None,
                                      self.chi_path.data_ptr(), self.reff.data_ptr(), N.stream_of(self.device)),
                 "fm_chi_paths")
         return self.chi_path

@@ -152,6 +152,38 @@ class EdgeTables:
     chw: np.ndarray = None      # (nchan, E) complex
     rotc: np.ndarray = None
     rotn: np.ndarray = None
+    gpaths: int = 4
+    gstart: list = field(default_factory=list)
+
+
+def energy_bands(profile: np.ndarray, nbands: int = 4):
+    """Paths per workgroup and the split of the energies into `nbands` contiguous bands of at most 64 / paths
+    energies each, minimising sum over bands of (band size) x (largest partial-wave count in the band): the lanes of
+    a wavefront then loop over similar numbers of partial waves."""
+    E = len(profile)
+    gp = 4 if E <= 64 else (2 if E <= 128 else 1)
+    cap = 64 // gp
+    if E > cap * nbands:
+        raise ValueError(f"{E} energies do not fit {nbands} bands of {cap}")
+    INF = float("inf")
+    # cost[i][j]: band [i, j)
+    best = [[INF] * (E + 1) for _ in range(nbands + 1)]
+    cut = [[0] * (E + 1) for _ in range(nbands + 1)]
+    best[0][0] = 0.0
+    for b in range(1, nbands + 1):
+        for j in range(E + 1):
+            for i in range(max(0, j - cap), j + 1):
+                if best[b - 1][i] == INF:
+                    continue
+                c = best[b - 1][i] + (float(profile[i:j].max()) * (j - i) if j > i else 0.0)
+                if c < best[b][j]:
+                    best[b][j], cut[b][j] = c, i
+    starts = [E]
+    j = E
+    for b in range(nbands, 0, -1):
+        j = cut[b][j]
+        starts.append(j)
+    return gp, starts[::-1]
 
 
 def edge_tables(P: PhaseData, E: int | None = None) -> EdgeTables:
@@ -183,9 +215,11 @@ def edge_tables(P: PhaseData, E: int | None = None) -> EdgeTables:
             if lind[k] == l:
                 chw[c] += bdiag[k] * P.rkk[:E, k - 1, 0] ** 2
     rotc, rotn = rotation_coefficients(L1)
+    gpaths, gstart = energy_bands(lm.max(1) + 1)
     return EdgeTables(E=E, L1=L1, nph1=nph1, ilinit=linit + 1, kinit=kinit, lmax=np.ascontiguousarray(lm.T),
                       lmax1=(lm.max(0) + 1).astype(np.int32), ck=ck, ckmxk=ck - xk, kf=xk / BOHR,
-                      kf_full=P.xk[:P.ne1] / BOHR, tl=tl, chan_il=[l + 1 for l in ls], chw=chw, rotc=rotc, rotn=rotn)
+                      kf_full=P.xk[:P.ne1] / BOHR, tl=tl, chan_il=[l + 1 for l in ls], chw=chw, rotc=rotc, rotn=rotn,
+                      gpaths=gpaths, gstart=gstart)
 
 
 def energies_needed(P: PhaseData, q) -> int:

@@ -48,6 +48,9 @@ typedef struct fm_edge {
     const double *chw64;
     const double *rotc;          /* [L1 + 1][49][3] d-recursion coefficients P, Q, R per l and (m1 + 3) * 7 + m2 + 3 */
     const double *rotn;          /* [L1 + 1][16] normalisation xnlm(il, |m1|) (-1)^|m1| / xnlm(il, |m2|), index |m1| * 4 + |m2| */
+    int32_t gpaths;              /* host: paths per workgroup (4 if E <= 64, 2 if E <= 128, else 1) */
+    int32_t gstart[5];           /* host: the 4 energy bands of a workgroup's waves, [gstart[w], gstart[w+1]), gstart[4] = E,
+                                    each at most 64 / gpaths energies (chosen so lanes of a band have similar lmax) */
 } fm_edge;
 
 /* A path list in the atom-index form of feffgpu's forward model (src/feffgpu/exafs/forward.py:143-163, 664-685). */
@@ -114,15 +117,17 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
  * order [npath] (device) is the processing order, a permutation of 0..npath-1 grouped into nbucket buckets:
  * bucket b holds order[bucket_offset[b] .. bucket_offset[b+1]) (host array), all of kind bucket_kind[b] (host):
  * FM_BUCKET_K_MS (K/L1 edge, 3..9 legs), FM_BUCKET_LM_MS (L2/L3/M edges, 3..9 legs), FM_BUCKET_K_SS (K/L1
- * edge, 2 legs); bucket_nlegmax[b] (host, may be NULL) is the largest nleg in the bucket (sizes LDS).
+ * edge, 2 legs); every path of bucket b has edge bucket_edge[b] (host, may be NULL: edge 0);
+ * bucket_nlegmax[b] (host, may be NULL) is the largest nleg in the bucket (sizes LDS).
  * seg_ptr [nseg + 1] and seg_paths [npath] (device) list the paths of each segment (CSR). chi_path (float for
  * precision 32, double for 64) is required: it is also the workspace of the deterministic segment sum.
  * chi_seg (double) and reff (double, A) may be NULL. */
 enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_K_SS = 2 };
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
                  int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,
-                 const int32_t *bucket_kind, const int32_t *bucket_nlegmax, int32_t nseg, const int32_t *seg_ptr,
-                 const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff, void *stream);
+                 const int32_t *bucket_kind, const int32_t *bucket_edge, const int32_t *bucket_nlegmax, int32_t nseg,
+                 const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff,
+                 void *stream);
 
 /* Adjoint of fm_chi_paths: the same path list, buckets and grid (grid.qorder/qstart required); gseg [nseg][nk] =
  * dL/dchi_seg (double) with segment [npath] the row of each path, gpath [npath][nk] = dL/dchi_path (element type by
@@ -132,8 +137,8 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
  * recomputed inside (nothing from the forward call is kept). */
 int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                           int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
-                          const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_nlegmax,
-                          const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
+                          const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
+                          const int32_t *bucket_nlegmax, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
                           void *workspace, size_t workspace_bytes, void *stream);
 
