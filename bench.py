@@ -129,6 +129,10 @@ def main():
     pos_t = torch.tensor(pos, device=dev, dtype=torch.float64)
     sync = torch.cuda.synchronize
     units = len(pl) * len(q)
+    # launch policy (paths per workgroup) measured on this workload before anything is timed
+    from feffgpu_amd.tune import tune
+    policy = tune(plan32, pos_t)
+    print(f"[bench] launch policy {policy}", file=sys.stderr, flush=True)
 
     # parity of this workload: FP32 fused against FP64 fused (equal to the FP64 oracle to ~1e-14)
     s32 = plan32.forward(pos_t)[0].cpu().numpy()
@@ -157,6 +161,12 @@ def main():
     grad_rel = float(np.abs(g32 - g64).max() / np.abs(g64).max())
     ms_fwd64 = timed(lambda: plan64.forward(pos_t), max(2, args.steps // 2), 1, sync) * 1e3
     print(f"[bench] fp64 fused forward {ms_fwd64:.3f} ms/step", file=sys.stderr, flush=True)
+    # the comparator: the plain FP64 pipeline (the reference's op sequence, one kernel per stage, HBM intermediates)
+    plain_seg = plan64.forward_plain(pos_t)[0].cpu().numpy()
+    plain_rel = float(np.abs(plain_seg - s64).max() / np.abs(s64).max())
+    ms_plain64 = timed(lambda: plan64.forward_plain(pos_t), 2, 1, sync) * 1e3
+    print(f"[bench] fp64 plain forward {ms_plain64:.3f} ms/step (vs fused fp64 max rel {plain_rel:.1e})",
+          file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: the bucket with the most work, timed alone with HIP events on its stream
     cm = path_cmacs(plan32.tables[0], plan32.nleg_h, pl.ipots)
@@ -192,7 +202,12 @@ def main():
                mode="fused FP32 forward + adjoint (k^2-weighted L2 loss per frame -> dL/dpositions)",
                fp32_fused_fwd=dict(value=units / (ms_fwd32 * 1e-3), ms_per_step=ms_fwd32),
                fp64_fused_fwd=dict(value=units / (ms_fwd64 * 1e-3), ms_per_step=ms_fwd64),
+               fp64_plain_fwd=dict(value=units / (ms_plain64 * 1e-3), ms_per_step=ms_plain64,
+                                   max_rel_vs_fused_fp64=plain_rel),
+               speedup_fp32_fused_fwd_bwd_vs_fp64_plain_fwd=ms_plain64 / ms_fb32,
                parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
+               launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
+                                  backward_ms=policy["backward"]),
                roofline=roof)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, pl, q, cell, args.cpu_seconds)

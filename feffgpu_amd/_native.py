@@ -24,11 +24,15 @@ _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 
 
+class FmBands(ctypes.Structure):
+    _fields_ = [("nband", _i32), ("start", _i32 * 17), ("wave", _i32 * 16)]
+
+
 class FmEdge(ctypes.Structure):
     _fields_ = [("E", _i32), ("L1", _i32), ("nph1", _i32), ("ilinit", _i32), ("nchan", _i32),
                 ("chan_il", _i32 * FM_MAX_CHAN), ("lmax1", _i32 * FM_MAX_POT),
                 ("lmax", _vp), ("ck", _vp), ("ckmxk", _vp), ("kf", _vp), ("tl32", _vp), ("tl64", _vp),
-                ("chw32", _vp), ("chw64", _vp), ("rotc", _vp), ("rotn", _vp), ("gpaths", _i32), ("gstart", _i32 * 5)]
+                ("chw32", _vp), ("chw64", _vp), ("rotc", _vp), ("rotn", _vp), ("bands", FmBands * 4)]
 
 
 class FmPaths(ctypes.Structure):
@@ -90,6 +94,14 @@ def lib():
                                             ctypes.POINTER(FmGrid), _i32, _i32, _vp, _i32, ctypes.POINTER(_i32),
                                             ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp,
                                             _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+        L.fm_plain_path_bytes.restype = ctypes.c_size_t
+        L.fm_plain_path_bytes.argtypes = [ctypes.POINTER(FmEdge), _i32]
+        L.fm_chi_paths_plain.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid),
+                                         _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                         ctypes.POINTER(_i32), ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         ctypes.c_size_t, _vp]
+        L.fm_set_lds_budget.restype = _i32
+        L.fm_set_lds_budget.argtypes = [_i32, _i32]
         _lib = L
     return _lib
 
@@ -109,9 +121,9 @@ def ptr(t) -> int | None:
 
 
 def backward_workspace(precision: int, nlegmax: int, npath: int, device, max_slots: int = 4096):
-    """A device buffer for the adjoints' per-path prefix products: enough wave slots for min(npath, max_slots)
-    resident paths (rounded up to the workgroup size)."""
-    slots = max(4, min(int(max_slots), -(-max(int(npath), 1) // 4) * 4))
+    """A device buffer for the adjoints' per-lane state: band slots for the workgroups in flight (a workgroup takes
+    one per band of its plan, at most 16), at most max_slots and no more than npath workgroups need."""
+    slots = max(16, min(int(max_slots), 16 * max(int(npath), 1)))
     nbytes = lib().fm_backward_workspace_bytes(int(precision), int(nlegmax), slots)
     return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
 
@@ -157,7 +169,11 @@ class DeviceEdge:
         s.lmax, s.ck, s.ckmxk, s.kf = (t.data_ptr() for t in (self.lmax, self.ck, self.ckmxk, self.kf))
         s.tl32, s.tl64, s.chw32, s.chw64 = (t.data_ptr() for t in (self.tl32, self.tl64, self.chw32, self.chw64))
         s.rotc, s.rotn = self.rotc.data_ptr(), self.rotn.data_ptr()
-        s.gpaths = int(T.gpaths)
-        for i, v in enumerate(T.gstart):
-            s.gstart[i] = int(v)
+        for g, (gp, starts, waves) in enumerate(T.bands):
+            bp = s.bands[g]
+            bp.nband = len(starts) - 1 if starts else 0
+            for i, v in enumerate(starts):
+                bp.start[i] = int(v)
+            for i, v in enumerate(waves):
+                bp.wave[i] = int(v)
         self.struct = s

@@ -2,16 +2,20 @@
 // reff) (chi_fine) and d loss / d positions for a path list (the fused entry), the gradient that feffgpu obtains by
 // autograd through src/feffgpu/stage.py:147-388 and the gather of src/feffgpu/exafs/forward.py:153-163.
 //
-// One wavefront per path, one lane per phase-grid energy, nothing kept from the forward call:
-//   1. leg geometry (FP64), rotation tables d^l(beta) and their beta-derivatives into LDS;
-//   2. pass A per energy: the amplitude with the prefix products Pre_{s+1} = F_s ... F_1 stored in LDS;
-//   3. the adjoint of the fine grid (mode 1) or the given dL/dcchi (mode 0): dL/dcchi per energy;
-//   4. pass B per energy, the chain walked backwards with U_s = T F_{n-1} ... F_{s+1}: every scattering matrix is
-//      streamed over the partial waves again with the forward-mode tangents of its two leg lengths (the Hankel
-//      recursions differentiated) and of its rotation angle (the d recursion differentiated), contracted on the fly
-//      with its adjoint conj(dL/dptrac) (Pre_s U_s)^T; eta enters analytically through exp(-i eta m);
-//   5. per-leg gradients reduced over the lanes (fixed order), then pushed through the FP64 geometry with dual
-//      numbers (one lane per coordinate), and for a path list gathered per atom in a fixed order.
+// The workgroup layout is the forward's: G = ed.gpaths paths and 4 wavefronts, wavefront w takes the energy band
+// [gstart[w], gstart[w+1]) of all G paths, lane (path j, energy i) = j * (64 / G) + i. Nothing is kept from the
+// forward call:
+//   1. per path (one wave): leg geometry (FP64), the rotation tables d^l(beta) and their beta-derivatives into LDS;
+//   2. per (path, energy) lane, pass A: the amplitude, with the scattering matrices F_s and the prefix products
+//      Pre_{s+1} = F_s ... F_1 stored in the lane's slice of the workspace;
+//   3. per path (one wave): the adjoint of the fine grid (path lists) or the given dL/dcchi: dL/dcchi per energy;
+//   4. per (path, energy) lane, pass B: the chain walked backwards with U_s = T F_{n-1} ... F_{s+1}; every scattering
+//      matrix is streamed over the partial waves again for the forward-mode tangents of its two leg lengths (the
+//      Hankel recursions differentiated) and of its rotation angle (the d recursion differentiated), contracted on
+//      the fly with its adjoint conj(dL/dptrac) (Pre_s U_s)^T; eta enters analytically through exp(-i eta m). Each
+//      lane writes its per-leg contributions once to its gradient slots (workspace);
+//   5. per path (one wave): the slots summed over the path's lanes in a fixed order, then pushed through the FP64
+//      geometry with dual numbers (one lane per coordinate); for a path list gathered per atom in a fixed order.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -24,14 +28,23 @@
 using namespace fm;
 
 int fm_set_err(int code, const char *msg);
+size_t fm_lds_budget(int kernel);
 
 namespace {
 
-__host__ __device__ constexpr int pre_elems(int set) { return lamdef(set).L * lamdef(set).L0; }   // one stored prefix product
+__host__ __device__ constexpr int pre_elems(int set) { return lamdef(set).L * lamdef(set).L0; }   // one prefix product
+constexpr int FM_UPARK = 36 + FM_NREP;                              // parked: U (L0 x L <= 6 x 6), then K' (eta weights)
+__host__ __device__ inline int nacc(int nlegmax) { return 4 * nlegmax + 2; }   // gradient slots per lane
+
+// Gradient slots of a lane: kind X (leg length through the gam leg), Y (leg length through the gamtl leg), B (beta),
+// H (eta), each for legs 1..nlegmax, then the two sums of the cfac term.
+enum { GK_X = 0, GK_Y = 1, GK_B = 2, GK_H = 3 };
 
 struct BArgs {
-    fm_edge ed[FM_MAX_EDGE];
-    int nedge, npath_bucket, first;
+    fm_edge ed;                 // the edge of every path of this launch
+    fm_bands bp;                // its band plan for gp paths per workgroup
+    int gp;
+    int npath_bucket, first, edge_id;
     const int32_t *order;
     PathSrc src;
     const void *gcchi;          // mode 0: [npath][E] complex
@@ -41,11 +54,11 @@ struct BArgs {
     const double *gseg;         // mode 1: [nseg][nk]
     const void *gpath;          // mode 1: [npath][nk] or NULL
     double *gcoord;             // [npath][nslot][3]
-    int nslot;
-    int ltab, emax, nk, nlegmax, wpb;
-    size_t wave_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_gq, off_gacc, off_gpar, off_tmp;
-    unsigned char *work;        // per wave slot: the prefix products (global memory, cache-resident)
-    size_t work_per_wave, work_bytes;
+    int nslot, ltab, nk;
+    int nlegmax;                // the leg count of every path of this launch
+    size_t path_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_red;
+    unsigned char *work;        // per wave slot: prefix products, parked U, gradient slots
+    size_t work_per_wave, work_off_F, work_off_U, work_off_acc, work_bytes;
 };
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }
@@ -55,28 +68,29 @@ int hip_err(hipError_t e, const char *where) {
     return fm_set_err(FM_EHIP, buf);
 }
 
-template <class T, int SET> __host__ void layout(BArgs &a) {
+template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed) {
     size_t o = align16(sizeof(PathLDS));
-    const size_t tab = align16(sizeof(T) * (size_t)a.nlegmax * a.ltab * FM_NREP);
+    const size_t tab = align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
     a.off_W = o; o += tab;
     a.off_Wd = o; o += tab;
-    a.off_cc = o; o += align16(sizeof(cx<T>) * a.emax);
-    a.off_amp = o; o += align16(sizeof(T) * a.emax);
-    a.off_ang = o; o += align16(sizeof(double) * a.emax);
-    a.off_cb = o; o += align16(sizeof(cx<T>) * a.emax);
-    a.off_gq = o; o += align16(sizeof(T) * 2 * (size_t)a.nk);
+    a.off_cc = o; o += align16(sizeof(cx<T>) * ed.E);
+    a.off_amp = o; o += align16(sizeof(T) * ed.E);
+    a.off_ang = o; o += align16(sizeof(double) * ed.E);
+    a.off_cb = o; o += align16(sizeof(cx<T>) * ed.E);
+    a.off_red = o; o += align16(sizeof(double) * (nacc(a.nlegmax) + 6 * FM_MAX_LEGS));
+    a.path_bytes = o;
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
-    a.work_per_wave = (sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE + 255) & ~size_t(255);
-    a.off_gacc = o; o += align16(sizeof(T) * 3 * (size_t)a.nlegmax * FM_WAVE);
-    a.off_gpar = o; o += align16(sizeof(double) * (3 * (size_t)a.nlegmax + 1));
-    a.off_tmp = o; o += align16(sizeof(double) * 3 * FM_MAX_LEGS);
-    a.wave_bytes = o;
+    a.work_off_F = sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE;
+    a.work_off_U = a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE;
+    a.work_off_acc = a.work_off_U + sizeof(cx<T>) * FM_UPARK * FM_WAVE;
+    a.work_per_wave = (a.work_off_acc + sizeof(T) * (size_t)nacc(a.nlegmax) * FM_WAVE + 255) & ~size_t(255);
 }
 
 // Derivative tables Wd[s][il][k] = d/dbeta d^{il-1}_{m1 m2}(beta_{s+1}) times the same normalisation as W: the
-// three-term recursion differentiated, d'^{l} = (P c - Q) d'^{l-1} - P sin(beta) d^{l-1} - R d'^{l-2}.
+// three-term recursion differentiated, d'^{l} = (P c - Q) d'^{l-1} - P sin(beta) d^{l-1} - R d'^{l-2}. The last leg
+// up to il = 4, as W.
 template <class T, int SET>
-__device__ void build_rotation_d(const fm_edge &ed, const PathLDS &g, int n, int ltab, T *Wd) {
+__device__ __forceinline__ void build_rotation_d(const fm_edge &ed, const PathLDS &g, int n, int ltab, T *Wd) {
     constexpr PairDef PD = pairdef(SET);
     const int lane = lane_id();
     for (int t = lane; t < n * FM_NREP; t += FM_WAVE) {
@@ -90,8 +104,9 @@ __device__ void build_rotation_d(const fm_edge &ed, const PathLDS &g, int n, int
         T *Ws = Wd + (size_t)s * ltab * FM_NREP;
         const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
         const int mm = (m1 + 3) * 7 + (m2 + 3), nn = am1 * 4 + am2;
+        const int rows = s == n - 1 ? (ed.L1 < 4 ? ed.L1 : 4) : ed.L1;
         double dlm1 = 0.0, dl = 0.0, tlm1 = 0.0, tl = 0.0;
-        for (int il = 1; il <= ed.L1; ++il) {
+        for (int il = 1; il <= rows; ++il) {
             const int l = il - 1;
             double d, t1;
             if (l <= 2) {
@@ -110,20 +125,10 @@ __device__ void build_rotation_d(const fm_edge &ed, const PathLDS &g, int n, int
     wave_sync();
 }
 
-template <class T> struct TLB;
-template <> struct TLB<float> {
-    static __device__ __forceinline__ const float *tl(const fm_edge &e) { return e.tl32; }
-    static __device__ __forceinline__ const float *chw(const fm_edge &e) { return e.chw32; }
-};
-template <> struct TLB<double> {
-    static __device__ __forceinline__ const double *tl(const fm_edge &e) { return e.tl64; }
-    static __device__ __forceinline__ const double *chw(const fm_edge &e) { return e.chw64; }
-};
-
-// One partial-wave row of the adjoint of a block: adds the primal row to Ft and the contractions of the tangents
-// with K to gX (gam leg length), gY (gamtl leg length), gB (rotation angle).
+// One partial-wave row of the adjoint of a block: the contractions of the row's tangents with K added to gX (gam
+// leg length), gY (gamtl leg length), gB (rotation angle).
 template <class T, int SET, int NR, int NC>
-__device__ __forceinline__ void bwd_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&K)[FM_NREP], const cx<T> (&xa)[3],
+__device__ __forceinline__ void bwd_row(const cx<T> (&K)[FM_NREP], const cx<T> (&xa)[3],
                                         const cx<T> (&xd)[3], cx<T> y0, cx<T> y1, cx<T> y0d, cx<T> y1d,
                                         const T *__restrict__ w, const T *__restrict__ wd, T &gX, T &gY, T &gB) {
     constexpr PairDef PD = pairdef(SET);
@@ -138,7 +143,6 @@ __device__ __forceinline__ void bwd_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&K)[F
     for (int k = 0; k < FM_NREP; ++k)
         if (PD.ri[k] < NR && PD.rj[k] < NC) {
             const int aa = PD.a[k] - 1, bb = PD.b[k] - 1;
-            rfma(Ft[k], Z[aa][bb], w[k]);
             rfma(Q[aa][bb], K[k], w[k]);
             rfma(Qd[aa][bb], K[k], wd[k]);
         }
@@ -154,140 +158,187 @@ __device__ __forceinline__ void bwd_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&K)[F
     gY += y0d.r * v0.r - y0d.i * v0.i + y1d.r * v1.r - y1d.i * v1.i;
 }
 
+// The adjoint of fblock (its tangents contracted with K): rows il = 1..ilx of the lane, t-matrix fetched one row
+// ahead.
 template <class T, int SET, int NR, int NC>
-__device__ __forceinline__ void fblock_bwd(cx<T> (&Ft)[FM_NREP], const cx<T> (&K)[FM_NREP], cx<T> ck, T rX, T rY,
-                                           int ilx, int ilmax, const T *__restrict__ Wl, const T *__restrict__ Wdl,
+__device__ __forceinline__ void fblock_bwd(const cx<T> (&K)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
+                                           const T *__restrict__ Wl, const T *__restrict__ Wdl,
                                            const T *__restrict__ tl, int tlstride, T &gX, T &gY, T &gB) {
-#pragma unroll
-    for (int k = 0; k < FM_NREP; ++k) Ft[k] = mk<T>(T(0), T(0));
     gX = gY = gB = T(0);
     Hank3T<T> X;
     Hank2T<T> Y;
     X.init(minus_i_over(ck * rX), rX);
     Y.init(minus_i_over(ck * rY), rY);
     cx<T> xa[3], xd[3], ya[2], yd[2];
-    const int rseed = ilmax < 4 ? ilmax : 4;
+    cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
+    const int rseed = ilx < 4 ? ilx : 4;
     for (int r = 1; r <= rseed; ++r) {
         X.seed_row(r, xa, xd);
         if (r < 4) Y.seed_row(r, ya, yd); else Y.step(r, ya, yd);
-        const cx<T> t = mk<T>(tl[(size_t)r * tlstride], tl[(size_t)r * tlstride + 1]);
-        if (r <= ilx)
-            bwd_row<T, SET, NR, NC>(Ft, K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
-                                    Wdl + (size_t)r * FM_NREP, gX, gY, gB);
+        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
+        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        bwd_row<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
+                                Wdl + (size_t)r * FM_NREP, gX, gY, gB);
+        t = tn;
     }
 #pragma unroll 1
-    for (int r = 5; r <= ilmax; ++r) {
+    for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa, xd);
         Y.step(r, ya, yd);
-        const cx<T> t = mk<T>(tl[(size_t)r * tlstride], tl[(size_t)r * tlstride + 1]);
-        if (r <= ilx)
-            bwd_row<T, SET, NR, NC>(Ft, K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
-                                    Wdl + (size_t)r * FM_NREP, gX, gY, gB);
+        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
+        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        bwd_row<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
+                                Wdl + (size_t)r * FM_NREP, gX, gY, gB);
+        t = tn;
     }
 }
 
-template <class T, int SET>
-__device__ __forceinline__ void phases_of(cx<T> (&ph)[6], const PathLDS &g, int J) {
-    constexpr LamDef D = lamdef(SET);
-    const T c1 = (T)g.c1[J], s1 = (T)g.s1[J], c2 = (T)g.c2[J], s2 = (T)g.s2[J];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        const int m = D.M[i];
-        ph[i] = m == 0 ? mk<T>(T(1), T(0)) : m == 1 ? mk<T>(c1, -s1) : m == -1 ? mk<T>(c1, s1) : m == 2 ? mk<T>(c2, -s2) : mk<T>(c2, s2);
-    }
+// Lane-private arrays in the workspace slice of one wavefront: element-major, lane-minor (coalesced), accessed
+// with buffer instructions. The slice's descriptor is wave-uniform (scalar registers), the lane enters as one
+// 32-bit offset and the element as a scalar offset, so no 64-bit address is formed per element. Accesses outside
+// the slice read 0 and write nothing.
+constexpr int FM_RSRC_WORD3 = 0x00020000;   // buffer descriptor word 3 on gfx9: 32-bit data format, no swizzle
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(unsigned char *p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, FM_RSRC_WORD3);
 }
+template <class E> struct LaneArray {
+    __amdgpu_buffer_rsrc_t rs;
+    uint32_t lo;           // lane * sizeof(E)
+    __device__ __forceinline__ E ld(int elem) const {
+        const int so = elem * (int)(FM_WAVE * sizeof(E));
+        E v;
+        if constexpr (sizeof(E) == 4) {
+            const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(rs, lo, so, 0);
+            __builtin_memcpy(&v, &x, 4);
+        } else if constexpr (sizeof(E) == 8) {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b64(rs, lo, so, 0);
+            __builtin_memcpy(&v, &x, 8);
+        } else {
+            static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
+            const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, lo, so, 0);
+            __builtin_memcpy(&v, &x, 16);
+        }
+        return v;
+    }
+    __device__ __forceinline__ void st(int elem, const E &v) const {
+        const int so = elem * (int)(FM_WAVE * sizeof(E));
+        if constexpr (sizeof(E) == 4) {
+            unsigned x;
+            __builtin_memcpy(&x, &v, 4);
+            __builtin_amdgcn_raw_buffer_store_b32(x, rs, lo, so, 0);
+        } else if constexpr (sizeof(E) == 8) {
+            __attribute__((ext_vector_type(2))) unsigned x;
+            __builtin_memcpy(&x, &v, 8);
+            __builtin_amdgcn_raw_buffer_store_b64(x, rs, lo, so, 0);
+        } else {
+            __attribute__((ext_vector_type(4))) unsigned x;
+            __builtin_memcpy(&x, &v, 16);
+            __builtin_amdgcn_raw_buffer_store_b128(x, rs, lo, so, 0);
+        }
+    }
+};
 
-// Per-lane accumulators of the per-leg gradients (kind 0: r, 1: beta, 2: eta; leg j = 1..n) in LDS, lane-private.
-template <class T> struct GAcc {
-    T *base;
+// Per-lane gradient slots (kind, leg j), each written once per path.
+template <class T> struct GSlots {
+    LaneArray<T> v;
     int nlegmax;
-    __device__ __forceinline__ void add(int kind, int j, T v) {
-        base[((size_t)kind * nlegmax + (j - 1)) * FM_WAVE + lane_id()] += v;
-    }
+    __device__ __forceinline__ void put(int kind, int j, T x) const { v.st(kind * nlegmax + j - 1, x); }
+    __device__ __forceinline__ void put_slot(int slot, T x) const { v.st(slot, x); }
 };
 
+// Prefix products (slot, element) and the FM_NREP representatives of F_s (in slot s - 1).
 template <class T, int SET> struct PreStore {
-    cx<T> *base;
-    __device__ __forceinline__ cx<T> &at(int slot, int elem) {
-        return base[((size_t)slot * pre_elems(SET) + elem) * FM_WAVE + lane_id()];
+    LaneArray<cx<T>> pre, fs;
+    __device__ __forceinline__ cx<T> get(int slot, int elem) const { return pre.ld(slot * pre_elems(SET) + elem); }
+    __device__ __forceinline__ void set(int slot, int elem, cx<T> v) const { pre.st(slot * pre_elems(SET) + elem, v); }
+    __device__ __forceinline__ void put_f(int s, const cx<T> (&Ft)[FM_NREP]) const {
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) fs.st((s - 1) * FM_NREP + k, Ft[k]);
+    }
+    __device__ __forceinline__ void get_f(int s, cx<T> (&Ft)[FM_NREP]) const {
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) Ft[k] = fs.ld((s - 1) * FM_NREP + k);
     }
 };
 
-// Pass A: the amplitude at energy e with the prefix products stored: slot s-2 holds Pre_s = F_{s-1} ... F_1
+// Pass A: the amplitude at energy ee with the prefix products stored: slot s-2 holds Pre_s = F_{s-1} ... F_1
 // (L x L0) for s = 2..n-1, slot max(n-2, 0) holds Pre_n (L0 x L0). Returns ptrac.
 template <class T, int SET>
-__device__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int ee, int n, PreStore<T, SET> pre) {
+__device__ __forceinline__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int ee, int n, PreStore<T, SET> pre) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
     const int E = ed.E;
     const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
     const int tlstride = 2 * E;
-    const T *tlbase = TLB<T>::tl(ed);
+    const T *tlbase = TL<T>::tl(ed);
     cx<T> Ft[FM_NREP], ph[6];
-    auto scat = [&](int s, int &ilx, int &ilmax, const T *&tl, const T *&Wl) {
+    auto scat = [&](int s, int &ilx, const T *&tl, const T *&Wl) {
         const int ip = g.ipot[s];
         ilx = ed.lmax[(size_t)ip * E + ee] + 1;
-        ilmax = ed.lmax1[ip];
         tl = tlbase + ((size_t)ip * ltab) * tlstride + 2 * ee;
         Wl = W + (size_t)(s - 1) * ltab * FM_NREP;
     };
     const int pn_slot = n > 2 ? n - 2 : 0;
     if (n == 2) {
-        int ilx, ilmax;
+        int ilx;
         const T *tl, *Wl;
-        scat(1, ilx, ilmax, tl, Wl);
-        fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, ilmax, Wl, tl, tlstride);
-        phases_of<T, SET>(ph, g, 2);
+        scat(1, ilx, tl, Wl);
+        fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
+        pre.put_f(1, Ft);
+        row_phases<T, SET>(ph, g, 2);
 #pragma unroll
         for (int a = 0; a < L0; ++a)
 #pragma unroll
-            for (int b = 0; b < L0; ++b) pre.at(pn_slot, a * L0 + b) = ph[a] * Ft[PD.idx[a][b]] * T(PD.sg[a][b]);
+            for (int b = 0; b < L0; ++b) pre.set(pn_slot, a * L0 + b, ph[a] * Ft[PD.idx[a][b]] * T(PD.sg[a][b]));
     } else {
         {   // Pre_2 = F_1 (L x L0)
-            int ilx, ilmax;
+            int ilx;
             const T *tl, *Wl;
-            scat(1, ilx, ilmax, tl, Wl);
-            fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, ilmax, Wl, tl, tlstride);
-            phases_of<T, SET>(ph, g, 2);
+            scat(1, ilx, tl, Wl);
+            fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
+            pre.put_f(1, Ft);
+            row_phases<T, SET>(ph, g, 2);
 #pragma unroll
             for (int i = 0; i < L; ++i)
 #pragma unroll
-                for (int b = 0; b < L0; ++b) pre.at(0, i * L0 + b) = ph[i] * Ft[PD.idx[i][b]] * T(PD.sg[i][b]);
+                for (int b = 0; b < L0; ++b) pre.set(0, i * L0 + b, ph[i] * Ft[PD.idx[i][b]] * T(PD.sg[i][b]));
         }
 #pragma unroll 1
         for (int s = 2; s <= n - 2; ++s) {   // Pre_{s+1} = F_s Pre_s
-            int ilx, ilmax;
+            int ilx;
             const T *tl, *Wl;
-            scat(s, ilx, ilmax, tl, Wl);
-            fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, ilmax, Wl, tl, tlstride);
-            phases_of<T, SET>(ph, g, s + 1);
+            scat(s, ilx, tl, Wl);
+            fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
+            pre.put_f(s, Ft);
+            row_phases<T, SET>(ph, g, s + 1);
 #pragma unroll
             for (int b = 0; b < L0; ++b) {
                 cx<T> col[L];
 #pragma unroll
-                for (int j = 0; j < L; ++j) col[j] = pre.at(s - 2, j * L0 + b);
+                for (int j = 0; j < L; ++j) col[j] = pre.get(s - 2, j * L0 + b);
 #pragma unroll
                 for (int i = 0; i < L; ++i) {
                     cx<T> acc = mk<T>(T(0), T(0));
 #pragma unroll
                     for (int j = 0; j < L; ++j) cfma(acc, Ft[PD.idx[i][j]] * T(PD.sg[i][j]), col[j]);
-                    pre.at(s - 1, i * L0 + b) = ph[i] * acc;
+                    pre.set(s - 1, i * L0 + b, ph[i] * acc);
                 }
             }
         }
         {   // Pre_n = F_{n-1} Pre_{n-1} (L0 x L0)
-            int ilx, ilmax;
+            int ilx;
             const T *tl, *Wl;
-            scat(n - 1, ilx, ilmax, tl, Wl);
-            fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, ilmax, Wl, tl, tlstride);
-            phases_of<T, SET>(ph, g, n);
+            scat(n - 1, ilx, tl, Wl);
+            fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
+            pre.put_f(n - 1, Ft);
+            row_phases<T, SET>(ph, g, n);
             cx<T> pn[L0][L0];
 #pragma unroll
             for (int b = 0; b < L0; ++b) {
                 cx<T> col[L];
 #pragma unroll
-                for (int j = 0; j < L; ++j) col[j] = pre.at(n - 3, j * L0 + b);
+                for (int j = 0; j < L; ++j) col[j] = pre.get(n - 3, j * L0 + b);
 #pragma unroll
                 for (int i = 0; i < L0; ++i) {
                     cx<T> acc = mk<T>(T(0), T(0));
@@ -299,7 +350,7 @@ __device__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, const T *W, int lta
 #pragma unroll
             for (int i = 0; i < L0; ++i)
 #pragma unroll
-                for (int b = 0; b < L0; ++b) pre.at(pn_slot, i * L0 + b) = pn[i][b];
+                for (int b = 0; b < L0; ++b) pre.set(pn_slot, i * L0 + b, pn[i][b]);
         }
     }
     // termination (primal) and the trace
@@ -313,7 +364,7 @@ __device__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, const T *W, int lta
 #pragma unroll
     for (int k = 0; k < FM_NREP; ++k) Tt[k] = mk<T>(T(0), T(0));
     const T *Wn = W + (size_t)(n - 1) * ltab * FM_NREP;
-    const T *chw = TLB<T>::chw(ed);
+    const T *chw = TL<T>::chw(ed);
     cx<T> xa[3], yb[2];
     for (int r = 1; r <= 4; ++r) {
         X.seed_row(r, xa, r <= lx1);
@@ -326,12 +377,12 @@ __device__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, const T *W, int lta
             accum_row<T, SET, L0, L0>(Tt, xa, yb[0] * w, yb[1] * w, Wn + (size_t)r * FM_NREP);
         }
     }
-    phases_of<T, SET>(ph, g, 1);
+    row_phases<T, SET>(ph, g, 1);
     cx<T> ptrac = mk<T>(T(0), T(0));
 #pragma unroll
     for (int a = 0; a < L0; ++a)
 #pragma unroll
-        for (int b = 0; b < L0; ++b) cfma(ptrac, ph[a] * Tt[PD.idx[a][b]] * T(PD.sg[a][b]), pre.at(pn_slot, b * L0 + a));
+        for (int b = 0; b < L0; ++b) cfma(ptrac, ph[a] * Tt[PD.idx[a][b]] * T(PD.sg[a][b]), pre.get(pn_slot, b * L0 + a));
     return ptrac;
 }
 
@@ -350,68 +401,79 @@ __device__ __forceinline__ cx<T> cfac_of(const fm_edge &ed, const PathLDS &g, in
 }
 
 // One step of pass B for scatterer s with an NR x NC block: H = conj(pbar) (Pre_s U_s)^T, its contraction with
-// the tangents of F_s, eta_{s+1}, and U <- U F_s (unless last).
-template <class T, int SET, int NR, int NC>
+// the tangents of F_s, eta_{s+1}, and U <- U F_s (unless last). With the register budget of 2 wavefronts per SIMD, U
+// is parked in the workspace while F_s is streamed and F_s is fetched afterwards; with the whole register file (1
+// per SIMD) both stay in registers and F_s is fetched before the stream, so its latency is hidden.
+template <class T, int SET, int NR, int NC, int WPE>
 __device__ __forceinline__ void chain_step(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab,
                                            int ee, int s, cx<T> pbc, cx<T> (&U)[lamdef(SET).L0][lamdef(SET).L],
-                                           PreStore<T, SET> pre, bool has_pre, bool last, GAcc<T> acc) {
+                                           PreStore<T, SET> pre, LaneArray<cx<T>> upark, bool has_pre, bool last,
+                                           GSlots<T> gs) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L0 = D.L0;
     const int E = ed.E;
     const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
     cx<T> ph[6];
-    phases_of<T, SET>(ph, g, s + 1);
-    // K[rep] = sum over the orbit of H[l1][l2] ph[l1] sg, H[l1][l2] = pbc * M[l2][l1], M = Pre_s U (NC x NR)
-    cx<T> K[FM_NREP];
+    row_phases<T, SET>(ph, g, s + 1);
+    // K[rep] = sum over the orbit of H[l1][l2] ph[l1] sg, H[l1][l2] = pbc * M[l2][l1], M = Pre_s U (NC x NR); Kp the
+    // same sum weighted by m_{l1}: dL/deta_{s+1} = Im sum_rep Kp[rep] Ft[rep]
+    cx<T> K[FM_NREP], Kp[FM_NREP];
 #pragma unroll
-    for (int k = 0; k < FM_NREP; ++k) K[k] = mk<T>(T(0), T(0));
+    for (int k = 0; k < FM_NREP; ++k) K[k] = Kp[k] = mk<T>(T(0), T(0));
 #pragma unroll
     for (int l2 = 0; l2 < NC; ++l2) {
         cx<T> prow[L0];
 #pragma unroll
         for (int a = 0; a < L0; ++a)
-            prow[a] = has_pre ? pre.at(s - 2, l2 * L0 + a) : (a == l2 ? mk<T>(T(1), T(0)) : mk<T>(T(0), T(0)));
+            prow[a] = has_pre ? pre.get(s - 2, l2 * L0 + a) : (a == l2 ? mk<T>(T(1), T(0)) : mk<T>(T(0), T(0)));
 #pragma unroll
         for (int l1 = 0; l1 < NR; ++l1) {
             cx<T> m = mk<T>(T(0), T(0));
 #pragma unroll
             for (int a = 0; a < L0; ++a) cfma(m, prow[a], U[a][l1]);
             const cx<T> h = pbc * m * ph[l1];
+            const T hs = T(PD.sg[l1][l2] * D.M[l1]);
             if (PD.sg[l1][l2] > 0) K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] + h;
             else K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] - h;
+            if (D.M[l1] != 0) rfma(Kp[PD.idx[l1][l2]], h, hs);
         }
     }
-    const int ip = g.ipot[s];
-    const int ilx = ed.lmax[(size_t)ip * E + ee] + 1, ilmax = ed.lmax1[ip];
-    const T *tl = TLB<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
     cx<T> Ft[FM_NREP];
-    T gX, gY, gB;
-    fblock_bwd<T, SET, NR, NC>(Ft, K, ck, (T)g.r[s + 1], (T)g.r[s], ilx, ilmax, W + (size_t)(s - 1) * ltab * FM_NREP,
-                               Wd + (size_t)(s - 1) * ltab * FM_NREP, tl, 2 * E, gX, gY, gB);
-    acc.add(0, s + 1, gX);
-    acc.add(0, s, gY);
-    acc.add(1, s, gB);
-    // eta_{s+1}: sum m_{l1} Im(H F) with F = ph sg Ft; recompute H from K's ingredients is costlier than this
-    // second pass over (l1, l2) using M again
-    T geta = T(0);
-#pragma unroll
-    for (int l2 = 0; l2 < NC; ++l2) {
-        cx<T> prow[L0];
+    if constexpr (WPE == 1) {
+        pre.get_f(s, Ft);
+    } else {
 #pragma unroll
         for (int a = 0; a < L0; ++a)
-            prow[a] = has_pre ? pre.at(s - 2, l2 * L0 + a) : (a == l2 ? mk<T>(T(1), T(0)) : mk<T>(T(0), T(0)));
 #pragma unroll
-        for (int l1 = 0; l1 < NR; ++l1) {
-            if (D.M[l1] == 0) continue;
-            cx<T> m = mk<T>(T(0), T(0));
+            for (int l1 = 0; l1 < NR; ++l1) upark.st(a * 6 + l1, U[a][l1]);
 #pragma unroll
-            for (int a = 0; a < L0; ++a) cfma(m, prow[a], U[a][l1]);
-            const cx<T> hf = pbc * m * ph[l1] * Ft[PD.idx[l1][l2]] * T(PD.sg[l1][l2]);
-            geta += T(D.M[l1]) * hf.i;
-        }
+        for (int k = 0; k < FM_NREP; ++k) upark.st(36 + k, Kp[k]);
     }
-    acc.add(2, s + 1, geta);
+    const int ip = g.ipot[s];
+    const int ilx = ed.lmax[(size_t)ip * E + ee] + 1;
+    const T *tl = TL<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
+    T gX, gY, gB;
+    fblock_bwd<T, SET, NR, NC>(K, ck, (T)g.r[s + 1], (T)g.r[s], ilx, W + (size_t)(s - 1) * ltab * FM_NREP,
+                               Wd + (size_t)(s - 1) * ltab * FM_NREP, tl, 2 * E, gX, gY, gB);
+    gs.put(GK_X, s + 1, gX);
+    gs.put(GK_Y, s, gY);
+    gs.put(GK_B, s, gB);
+    if constexpr (WPE != 1) {
+        pre.get_f(s, Ft);
+#pragma unroll
+        for (int a = 0; a < L0; ++a)
+#pragma unroll
+            for (int l1 = 0; l1 < NR; ++l1) U[a][l1] = upark.ld(a * 6 + l1);
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) Kp[k] = upark.ld(36 + k);
+    }
+    // eta_{s+1}: sum over (l1, l2) of m_{l1} Im(H F), F = ph sg Ft
+    T geta = T(0);
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k)
+        if (PD.ri[k] < NR && PD.rj[k] < NC) geta += Kp[k].r * Ft[k].i + Kp[k].i * Ft[k].r;
+    gs.put(GK_H, s + 1, geta);
     if (!last) {
 #pragma unroll
         for (int a = 0; a < L0; ++a) {
@@ -435,20 +497,20 @@ __device__ __forceinline__ void chain_step(const fm_edge &ed, const PathLDS &g, 
 }
 
 // Pass B at energy ee: cbar = dL/dcchi, cc = cchi.
-template <class T, int SET>
-__device__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee, int n,
-                       cx<T> cbar, cx<T> cc, PreStore<T, SET> pre, GAcc<T> acc) {
+template <class T, int SET, int WPE>
+__device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee, int n,
+                       cx<T> cbar, cx<T> cc, PreStore<T, SET> pre, LaneArray<cx<T>> upark, GSlots<T> gs) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
     const int E = ed.E;
-    // cfac's own dependence on the leg lengths: d cfac / d r_j = cfac (i (ck - xk) - 1 / r_j)
+    // cfac's own dependence on the leg lengths: d cfac / d r_j = cfac (i (ck - xk) - 1 / r_j), i.e. a1 - a2 / r_j
     {
         const cx<T> w = conj(cbar) * cc;
         const cx<T> dk = mk<T>((T)ed.ckmxk[2 * ee], (T)ed.ckmxk[2 * ee + 1]);
         const cx<T> wdk = w * dk;
-        const T a1 = -wdk.i, a2 = w.r;
-        for (int j = 1; j <= n; ++j) acc.add(0, j, a1 - a2 / (T)g.r[j]);
+        gs.put_slot(4 * gs.nlegmax, -wdk.i);
+        gs.put_slot(4 * gs.nlegmax + 1, w.r);
     }
     const cx<T> cf = cfac_of<T>(ed, g, ee, n);
     const cx<T> pbc = conj(cbar * conj(cf));          // conj(dL/dptrac)
@@ -458,7 +520,7 @@ __device__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T 
     cx<T> U[L0][L];
     {
         cx<T> ph[6];
-        phases_of<T, SET>(ph, g, 1);
+        row_phases<T, SET>(ph, g, 1);
         cx<T> K[FM_NREP];
 #pragma unroll
         for (int k = 0; k < FM_NREP; ++k) K[k] = mk<T>(T(0), T(0));
@@ -466,7 +528,7 @@ __device__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T 
         for (int a = 0; a < L0; ++a)
 #pragma unroll
             for (int b = 0; b < L0; ++b) {
-                const cx<T> h = pbc * pre.at(pn_slot, b * L0 + a) * ph[a];
+                const cx<T> h = pbc * pre.get(pn_slot, b * L0 + a) * ph[a];
                 if (PD.sg[a][b] > 0) K[PD.idx[a][b]] = K[PD.idx[a][b]] + h;
                 else K[PD.idx[a][b]] = K[PD.idx[a][b]] - h;
             }
@@ -481,7 +543,7 @@ __device__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T 
         for (int k = 0; k < FM_NREP; ++k) Tt[k] = mk<T>(T(0), T(0));
         const T *Wn = W + (size_t)(n - 1) * ltab * FM_NREP;
         const T *Wdn = Wd + (size_t)(n - 1) * ltab * FM_NREP;
-        const T *chw = TLB<T>::chw(ed);
+        const T *chw = TL<T>::chw(ed);
         T gX = T(0), gY = T(0), gB = T(0);
         cx<T> xa[3], xd[3], ya[2], yd[2];
         for (int r = 1; r <= 4; ++r) {
@@ -492,13 +554,14 @@ __device__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T 
                 if (ed.chan_il[q] == r) c = q;
             if (c >= 0) {
                 const cx<T> w = mk<T>(chw[((size_t)c * E + ee) * 2], chw[((size_t)c * E + ee) * 2 + 1]) * T(2 * r - 1);
-                bwd_row<T, SET, L0, L0>(Tt, K, xa, xd, ya[0] * w, ya[1] * w, yd[0] * w, yd[1] * w,
+                accum_row<T, SET, L0, L0>(Tt, xa, ya[0] * w, ya[1] * w, Wn + (size_t)r * FM_NREP);
+                bwd_row<T, SET, L0, L0>(K, xa, xd, ya[0] * w, ya[1] * w, yd[0] * w, yd[1] * w,
                                         Wn + (size_t)r * FM_NREP, Wdn + (size_t)r * FM_NREP, gX, gY, gB);
             }
         }
-        acc.add(0, 1, gX);
-        acc.add(0, n, gY);
-        acc.add(1, n, gB);
+        gs.put(GK_X, 1, gX);
+        gs.put(GK_Y, n, gY);
+        gs.put(GK_B, n, gB);
         T geta = T(0);
 #pragma unroll
         for (int a = 0; a < L0; ++a)
@@ -507,20 +570,21 @@ __device__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T 
                 const cx<T> t = (b < L0) ? ph[a] * Tt[PD.idx[a][b]] * T(PD.sg[a][b]) : mk<T>(T(0), T(0));
                 U[a][b] = t;
                 if (b < L0 && D.M[a] != 0) {
-                    const cx<T> hf = pbc * pre.at(pn_slot, b * L0 + a) * t;
+                    const cx<T> hf = pbc * pre.get(pn_slot, b * L0 + a) * t;
                     geta += T(D.M[a]) * hf.i;
                 }
             }
-        acc.add(2, 1, geta);
+        gs.put(GK_H, 1, geta);
     }
     // ---- the chain, s = n-1 .. 1
     if (n == 2) {
-        chain_step<T, SET, L0, L0>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, false, true, acc);
+        chain_step<T, SET, L0, L0, WPE>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, upark, false, true, gs);
     } else {
-        chain_step<T, SET, L0, L>(ed, g, W, Wd, ltab, ee, n - 1, pbc, U, pre, true, false, acc);
+        chain_step<T, SET, L0, L, WPE>(ed, g, W, Wd, ltab, ee, n - 1, pbc, U, pre, upark, true, false, gs);
 #pragma unroll 1
-        for (int s = n - 2; s >= 2; --s) chain_step<T, SET, L, L>(ed, g, W, Wd, ltab, ee, s, pbc, U, pre, true, false, acc);
-        chain_step<T, SET, L, L0>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, false, true, acc);
+        for (int s = n - 2; s >= 2; --s)
+            chain_step<T, SET, L, L, WPE>(ed, g, W, Wd, ltab, ee, s, pbc, U, pre, upark, true, false, gs);
+        chain_step<T, SET, L, L0, WPE>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, upark, false, true, gs);
     }
 }
 
@@ -531,36 +595,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Adjoint of the fine grid for the wave's path: from gq (dL/dA(q), dL/dPhi(q) already in LDS) to dL/dcchi per
-// energy, gathered over the q points that interpolate from each energy (qorder / qstart runs, fixed order).
-template <class T>
-__device__ void fine_grid_gather(int E, const int32_t *qorder, const int32_t *qstart, const double *fq, const T *gA,
-                                 const T *gP, const cx<T> *cc, const T *amp, cx<T> *cb) {
-    const int lane = lane_id();
-    for (int e = lane; e < E; e += FM_WAVE) {
-        double ab = 0.0, pb = 0.0;
-        for (int t = qstart[e]; t < qstart[e + 1]; ++t) {
-            const int i = qorder[t];
-            ab += (1.0 - fq[i]) * (double)gA[i];
-            pb += (1.0 - fq[i]) * (double)gP[i];
-        }
-        if (e > 0)
-            for (int t = qstart[e - 1]; t < qstart[e]; ++t) {
-                const int i = qorder[t];
-                ab += fq[i] * (double)gA[i];
-                pb += fq[i] * (double)gP[i];
-            }
-        const double cr = (double)cc[e].r, ci = (double)cc[e].i;
-        const double c2 = cr * cr + ci * ci, a = (double)amp[e];
-        cx<T> v = mk<T>(T(0), T(0));
-        if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
-        cb[e] = v;
-    }
-}
-
 // amp / unwrapped phase of cc[0..E) into amp[], ang[] (the forward epilogue's first half)
 template <class T>
-__device__ void amp_phase(int E, const cx<T> *cc, T *amp, double *ang) {
+__device__ __forceinline__ void amp_phase(int E, const cx<T> *cc, T *amp, double *ang) {
     const int lane = lane_id();
     for (int e = lane; e < E; e += FM_WAVE) {
         amp[e] = (T)sqrt((double)cc[e].r * cc[e].r + (double)cc[e].i * cc[e].i + 1e-30);
@@ -592,14 +629,17 @@ __device__ void amp_phase(int E, const cx<T> *cc, T *amp, double *ang) {
     wave_sync();
 }
 
-// dL/dA(q) and dL/dPhi(q) from dL/dchi(q) (chi = A sin(Phi + 2 q reff) dw); returns dL/dreff (A), wave-summed.
+// Adjoint of the fine grid of one path (one wave): chi(q) = A(q) sin(Phi(q) + 2 q reff) exp(-2 sigma2 q^2), A and
+// Phi linear in amp[] and the unwrapped ang[]. From dL/dchi(q) = gup_scale gup_d[q] + gup_p[q] to dL/dcchi per
+// energy: each energy gathers the q points that interpolate from it (qorder / qstart runs, in a fixed order), and
+// dL/dA(q), dL/dPhi(q) are evaluated where they are used. Returns dL/dreff (A), wave-summed.
 template <class T>
-__device__ double fine_grid_adjoint_q(const int32_t *jq, const double *fq, const double *q, int nk, double reff_A,
-                                      double sigma2, const T *amp, const double *ang, const double *gup_d,
-                                      double gup_scale, const T *gup_p, T *gA, T *gP) {
+__device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, const double *fq, const double *q, const int32_t *qorder,
+                                    const int32_t *qstart, double reff_A, double sigma2, const double *gup_d,
+                                    double gup_scale, const T *gup_p, const cx<T> *cc, const T *amp, const double *ang,
+                                    cx<T> *cb) {
     const int lane = lane_id();
-    double part = 0.0;
-    for (int i = lane; i < nk; i += FM_WAVE) {
+    auto adj = [&](int i, double &gA, double &gP) {
         const int j = jq[i];
         const double f = fq[i];
         const double A = (1.0 - f) * (double)amp[j] + f * (double)amp[j + 1];
@@ -611,124 +651,216 @@ __device__ double fine_grid_adjoint_q(const int32_t *jq, const double *fq, const
         if (sigma2 != 0.0) gu *= exp(-2.0 * sigma2 * q[i] * q[i]);
         T sn, cs;
         if constexpr (sizeof(T) == 4) sincosf((float)ph, &sn, &cs); else sincos(ph, &sn, &cs);
-        gA[i] = (T)(gu * (double)sn);
-        const T gp = (T)(gu * A * (double)cs);
-        gP[i] = gp;
-        part += 2.0 * q[i] * (double)gp;
+        gA = (double)(T)(gu * (double)sn);
+        gP = (double)(T)(gu * A * (double)cs);
+    };
+    double rb = 0.0;
+    for (int e = lane; e < E; e += FM_WAVE) {
+        double ab = 0.0, pb = 0.0;
+        for (int t = qstart[e]; t < qstart[e + 1]; ++t) {    // q points with lower interpolation energy e
+            const int i = qorder[t];
+            double gA, gP;
+            adj(i, gA, gP);
+            ab += (1.0 - fq[i]) * gA;
+            pb += (1.0 - fq[i]) * gP;
+            rb += 2.0 * q[i] * gP;
+        }
+        if (e > 0)
+            for (int t = qstart[e - 1]; t < qstart[e]; ++t) {  // ... and upper interpolation energy e
+                const int i = qorder[t];
+                double gA, gP;
+                adj(i, gA, gP);
+                ab += fq[i] * gA;
+                pb += fq[i] * gP;
+            }
+        const double cr = (double)cc[e].r, ci = (double)cc[e].i;
+        const double c2 = cr * cr + ci * ci, a = (double)amp[e];
+        cx<T> v = mk<T>(T(0), T(0));
+        if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
+        cb[e] = v;
     }
-    return wave_sum(part);
+    return wave_sum(rb);
 }
 
-template <class T, int SET>
-__global__ __launch_bounds__(FM_WAVE *WPB_MAX) __attribute__((amdgpu_waves_per_eu(1))) void k_backward(BArgs a) {
+template <class T, int SET, int WPE>
+__global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_backward(BArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int wid = threadIdx.x / FM_WAVE;
+    __shared__ int pidx[8], pval[8];
+    __shared__ double rbar[8];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
     const int lane = lane_id();
-    unsigned char *base = smem + (size_t)wid * a.wave_bytes;
-    PathLDS &g = *reinterpret_cast<PathLDS *>(base);
-    T *W = reinterpret_cast<T *>(base + a.off_W);
-    T *Wd = reinterpret_cast<T *>(base + a.off_Wd);
-    cx<T> *cc = reinterpret_cast<cx<T> *>(base + a.off_cc);
-    T *amp = reinterpret_cast<T *>(base + a.off_amp);
-    double *ang = reinterpret_cast<double *>(base + a.off_ang);
-    cx<T> *cb = reinterpret_cast<cx<T> *>(base + a.off_cb);
-    T *gA = reinterpret_cast<T *>(base + a.off_gq);
-    T *gP = gA + a.nk;
-    PreStore<T, SET> pre{reinterpret_cast<cx<T> *>(a.work + (size_t)(blockIdx.x * a.wpb + wid) * a.work_per_wave)};
-    GAcc<T> acc{reinterpret_cast<T *>(base + a.off_gacc), a.nlegmax};
-    double *gpar = reinterpret_cast<double *>(base + a.off_gpar);
-    double *tmp = reinterpret_cast<double *>(base + a.off_tmp);
-    for (int slot = blockIdx.x * a.wpb + wid; slot < a.npath_bucket; slot += gridDim.x * a.wpb) {
-        const int p = a.order ? a.order[a.first + slot] : a.first + slot;
-        int ei;
-        const int n = load_path(a.src, p, g, ei);
-        const fm_edge &ed = a.ed[ei];
-        const int ltab = ed.L1 + 1;
-        const int E = ed.E;
-        build_rotation<T, SET>(ed, g, n, ltab, W);
-        build_rotation_d<T, SET>(ed, g, n, ltab, Wd);
-        for (int k = 0; k < 3 * a.nlegmax; ++k) acc.base[(size_t)k * FM_WAVE + lane] = T(0);
-        const bool single = E <= FM_WAVE;
-        // 1-2: amplitude per energy (prefix products kept when one chunk covers every energy)
-        for (int e0 = 0; e0 < E; e0 += FM_WAVE) {
-            const int e = e0 + lane, ee = e < E ? e : E - 1;
-            const cx<T> pt = pass_A<T, SET>(ed, g, W, ltab, ee, n, pre);
-            const cx<T> c = pt * cfac_of<T>(ed, g, ee, n);
-            if (e < E) cc[e] = c;
-        }
-        wave_sync();
-        // 3: dL/dcchi
-        double reffbar = 0.0;
-        if (a.src.mode == 0) {
-            const T *gin = reinterpret_cast<const T *>(a.gcchi) + (size_t)p * E * 2;
-            for (int e = lane; e < E; e += FM_WAVE) cb[e] = mk<T>(gin[2 * e], gin[2 * e + 1]);
-            reffbar = a.greff ? a.greff[p] : 0.0;
-        } else {
-            amp_phase<T>(E, cc, amp, ang);
-            const double reff_A = g.reff * FM_BOHR;
-            const double scale = a.src.P.deg[p];
-            const double *gs = a.gseg ? a.gseg + (size_t)a.segment[p] * a.nk : nullptr;
-            const T *gpp = a.gpath ? reinterpret_cast<const T *>(a.gpath) + (size_t)p * a.nk : nullptr;
-            reffbar = fine_grid_adjoint_q<T>(a.G.jq + (size_t)ei * a.nk, a.G.fq + (size_t)ei * a.nk, a.G.q, a.nk, reff_A,
-                                             a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, amp, ang, gs, scale, gpp, gA, gP);
-            wave_sync();
-            fine_grid_gather<T>(E, a.G.qorder, a.G.qstart + (size_t)ei * a.G.qstride, a.G.fq + (size_t)ei * a.nk, gA, gP, cc,
-                                amp, cb);
-        }
-        wave_sync();
-        // 4: pass B (pass A again per chunk when the energies did not fit one chunk)
-        for (int e0 = 0; e0 < E; e0 += FM_WAVE) {
-            const int e = e0 + lane, ee = e < E ? e : E - 1;
-            if (!single) pass_A<T, SET>(ed, g, W, ltab, ee, n, pre);
-            cx<T> cbar = cb[ee], cce = cc[ee];
-            if (e >= E) cbar = mk<T>(T(0), T(0));
-            pass_B<T, SET>(ed, g, W, Wd, ltab, ee, n, cbar, cce, pre, acc);
-        }
-        wave_sync();
-        // 5: per-leg gradients summed over the lanes (fixed order), plus reff = sum r / 2
-        if (lane < 3 * n) {
-            const int kind = lane / n, j = lane % n;
-            double s = 0.0;
-            const T *row = acc.base + ((size_t)kind * a.nlegmax + j) * FM_WAVE;
-            for (int l = 0; l < FM_WAVE; ++l) s += (double)row[l];
-            if (kind == 0) s += 0.5 * FM_BOHR * reffbar;
-            gpar[lane] = s;
-        }
-        wave_sync();
-        // geometry adjoint: one lane per coordinate direction (bohr), dual numbers through every leg
-        const int ndir = a.src.mode == 0 ? 3 * n : 3 * (n - 1);
-        double gout = 0.0;
-        if (lane < ndir) {
-            const int atom = a.src.mode == 0 ? lane / 3 : lane / 3 + 1, comp = lane % 3;
-            Dd rat[FM_MAX_LEGS + 1][3];
-            for (int j = 0; j <= n; ++j)
-                for (int c = 0; c < 3; ++c) rat[j][c] = dd(g.rat[j][c], 0.0);
-            rat[atom][comp].d = 1.0;
-            if (atom == 0) rat[n][comp].d = 1.0;
-            double acc_g = 0.0;
-            for (int J = 1; J <= n; ++J) {
-                const LegGeoD lg = leg_geometry_dual(rat, J, n);
-                const int next = J == n ? 1 : J + 1;
-                acc_g += gpar[J - 1] * lg.r.d + gpar[n + J - 1] * lg.beta.d + gpar[2 * n + J - 1] * lg.gamma.d +
-                         gpar[2 * n + next - 1] * lg.alpha.d;
+    const fm_edge &ed = a.ed;
+    const int G = a.gp, cap = FM_WAVE / G;
+    const int ltab = ed.L1 + 1, E = ed.E;
+    const int n = a.nlegmax;
+    const int nband = a.bp.nband;
+    const uint32_t loc = (uint32_t)lane * sizeof(cx<T>), lor = (uint32_t)lane * sizeof(T);
+    // this lane's path and energy within a band, and that path's LDS
+    const int jl = lane / cap, il = lane % cap;
+    unsigned char *pbase = smem + (size_t)jl * a.path_bytes;
+    const PathLDS &gj = *reinterpret_cast<const PathLDS *>(pbase);
+    const T *Wj = reinterpret_cast<const T *>(pbase + a.off_W);
+    const T *Wdj = reinterpret_cast<const T *>(pbase + a.off_Wd);
+    cx<T> *ccj = reinterpret_cast<cx<T> *>(pbase + a.off_cc);
+    const cx<T> *cbj = reinterpret_cast<const cx<T> *>(pbase + a.off_cb);
+    unsigned char *const wg_work = a.work + (size_t)blockIdx.x * nband * a.work_per_wave;   // band b: + b * work_per_wave
+    const size_t wpw = a.work_per_wave;
+
+    for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
+        // ---- 1. setup: one wavefront per path
+        for (int j = wid; j < G; j += 4) {
+            const int slot = blk * G + j;
+            const bool valid = slot < a.npath_bucket;
+            const int s0 = valid ? slot : blk * G;
+            const int p = a.order ? a.order[a.first + s0] : a.first + s0;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
+            PathLDS &g = *reinterpret_cast<PathLDS *>(base);
+            int ei;
+            load_path(a.src, p, g, ei, n);
+            build_rotation<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_W));
+            build_rotation_d<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_Wd));
+            if (lane == 0) {
+                pidx[j] = p;
+                pval[j] = valid ? 1 : 0;
+                rbar[j] = (a.src.mode == 0 && a.greff) ? a.greff[p] : 0.0;
             }
-            gout = acc_g / FM_BOHR;
-            tmp[lane] = gout;
         }
-        wave_sync();
-        double *out = a.gcoord + (size_t)p * a.nslot * 3;
-        if (a.src.mode == 0) {
-            if (lane < 3 * n) out[lane] = gout;
-        } else {
-            if (lane < 3 * (n - 1)) out[3 + lane] = gout;
-            if (lane < 3) {
+        __syncthreads();
+        // ---- 2. pass A over this wavefront's bands
+        for (int b = 0; b < nband; ++b) {
+            if (a.bp.wave[b] != wid) continue;
+            unsigned char *wb = wg_work + (size_t)b * wpw;
+            const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
+                                       {slice_rsrc(wb + a.work_off_F, a.work_off_U - a.work_off_F), loc}};
+            const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
+            const int e = e0 + (il < nb ? il : nb - 1);
+            const cx<T> pt = pass_A<T, SET>(ed, gj, Wj, ltab, e, n, pre);
+            const cx<T> c = pt * cfac_of<T>(ed, gj, e, n);
+            if (il < nb) ccj[e] = c;
+        }
+        __syncthreads();
+        // ---- 3. dL/dcchi per energy (path lists: the fine grid's adjoint), one wavefront per path
+        if (a.src.mode == 1)
+            for (int j = wid; j < G; j += 4) {
+                if (!pval[j]) continue;
+                unsigned char *base = smem + (size_t)j * a.path_bytes;
+                const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+                const cx<T> *cc = reinterpret_cast<const cx<T> *>(base + a.off_cc);
+                T *amp = reinterpret_cast<T *>(base + a.off_amp);
+                double *ang = reinterpret_cast<double *>(base + a.off_ang);
+                cx<T> *cb = reinterpret_cast<cx<T> *>(base + a.off_cb);
+                const int p = pidx[j];
+                amp_phase<T>(E, cc, amp, ang);
+                const double *gsr = a.gseg ? a.gseg + (size_t)a.segment[p] * a.nk : nullptr;
+                const T *gpr = a.gpath ? reinterpret_cast<const T *>(a.gpath) + (size_t)p * a.nk : nullptr;
+                const double rb = fine_grid_adjoint<T>(E, a.G.jq + (size_t)a.edge_id * a.nk, a.G.fq + (size_t)a.edge_id * a.nk,
+                                                       a.G.q, a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride,
+                                                       g.reff * FM_BOHR, a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr,
+                                                       a.src.P.deg[p], gpr, cc, amp, ang, cb);
+                if (lane == 0) rbar[j] = rb;
+            }
+        __syncthreads();
+        // ---- 4. pass B over this wavefront's bands
+        for (int b = 0; b < nband; ++b) {
+            if (a.bp.wave[b] != wid) continue;
+            unsigned char *wb = wg_work + (size_t)b * wpw;
+            const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
+                                       {slice_rsrc(wb + a.work_off_F, a.work_off_U - a.work_off_F), loc}};
+            const LaneArray<cx<T>> upark{slice_rsrc(wb + a.work_off_U, a.work_off_acc - a.work_off_U), loc};
+            const GSlots<T> gs{{slice_rsrc(wb + a.work_off_acc, a.work_per_wave - a.work_off_acc), lor}, n};
+            const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
+            const int e = e0 + (il < nb ? il : nb - 1);
+            cx<T> cbar = mk<T>(T(0), T(0));
+            const cx<T> cce = ccj[e];
+            if (il < nb && pval[jl]) {
+                if (a.src.mode == 0) {
+                    const T *gin = reinterpret_cast<const T *>(a.gcchi) + ((size_t)pidx[jl] * E + e) * 2;
+                    cbar = mk<T>(gin[0], gin[1]);
+                } else {
+                    cbar = cbj[e];
+                }
+            }
+            pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, upark, gs);
+        }
+        __syncthreads();
+        // ---- 5. per path: the gradient slots summed over the path's lanes of every band (fixed order), the geometry
+        // adjoint, the output
+        for (int j = wid; j < G; j += 4) {
+            if (!pval[j]) continue;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
+            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+            double *red = reinterpret_cast<double *>(base + a.off_red);
+            double *gp = red + nacc(a.nlegmax);
+            double *tmp = gp + 3 * FM_MAX_LEGS;
+            const int nlm = n;
+            const int nq = nacc(nlm);
+            if (lane < nq) {
                 double s = 0.0;
-                for (int j = 0; j < n - 1; ++j) s += tmp[3 * j + lane];
-                out[lane] = -s;
+                for (int b = 0; b < nband; ++b) {
+                    const int nb = a.bp.start[b + 1] - a.bp.start[b];
+                    const T *src = reinterpret_cast<const T *>(wg_work + (size_t)b * wpw + a.work_off_acc) + (size_t)lane * FM_WAVE + j * cap;
+                    for (int i = 0; i < nb; ++i) s += (double)src[i];
+                }
+                red[lane] = s;
             }
-            if (lane >= 3 * n && lane < 3 * a.nslot) out[lane] = 0.0;
+            wave_sync();
+            // per-leg gradients: r_j (bohr) from both leg factors, the cfac term and reff = sum r / 2; beta_j; eta_j
+            if (lane < 3 * n) {
+                const int kind = lane / n, jg = lane % n + 1;
+                double v;
+                if (kind == 0)
+                    v = red[GK_X * nlm + jg - 1] + red[GK_Y * nlm + jg - 1] + red[4 * nlm] - red[4 * nlm + 1] / g.r[jg] +
+                        0.5 * FM_BOHR * rbar[j];
+                else if (kind == 1)
+                    v = red[GK_B * nlm + jg - 1];
+                else
+                    v = red[GK_H * nlm + jg - 1];
+                gp[lane] = v;
+            }
+            wave_sync();
+            // geometry adjoint: one lane per coordinate direction (bohr), dual numbers through every leg
+            const int ndir = a.src.mode == 0 ? 3 * n : 3 * (n - 1);
+            double gout = 0.0;
+            if (lane < ndir) {
+                const int atom = a.src.mode == 0 ? lane / 3 : lane / 3 + 1, comp = lane % 3;
+                auto point = [&](int idx, Dd (&v)[3]) {
+                    const int ai = idx == n ? 0 : idx;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[c] = dd(g.rat[idx][c], (ai == atom && c == comp) ? 1.0 : 0.0);
+                };
+                double acc_g = 0.0;
+                for (int J = 1; J <= n; ++J) {
+                    const int i = J == n ? 0 : J, ip1 = J == n ? 1 : J + 1, im1 = J - 1;
+                    Dd pm[3], pi[3], pp[3];
+                    point(im1, pm);
+                    point(i, pi);
+                    point(ip1, pp);
+                    const LegGeoD lg = leg_geometry_dual(pm, pi, pp);
+                    const int next = J == n ? 1 : J + 1;
+                    acc_g += gp[J - 1] * lg.r.d + gp[n + J - 1] * lg.beta.d + gp[2 * n + J - 1] * lg.gamma.d +
+                             gp[2 * n + next - 1] * lg.alpha.d;
+                }
+                gout = acc_g / FM_BOHR;
+                if (!g.ok) gout = __builtin_nan("");
+                tmp[lane] = gout;
+            }
+            wave_sync();
+            const int p = pidx[j];
+            double *out = a.gcoord + (size_t)p * a.nslot * 3;
+            if (a.src.mode == 0) {
+                if (lane < 3 * n) out[lane] = gout;
+            } else {
+                if (lane < 3 * (n - 1)) out[3 + lane] = gout;
+                if (lane < 3) {
+                    double s = 0.0;
+                    for (int k = 0; k < n - 1; ++k) s += tmp[3 * k + lane];
+                    out[lane] = -s;
+                }
+                if (lane >= 3 * n && lane < 3 * a.nslot) out[lane] = 0.0;
+            }
+            wave_sync();
         }
-        wave_sync();
+        __syncthreads();
     }
 }
 
@@ -756,16 +888,12 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, i
     T *amp = reinterpret_cast<T *>(base + align16(sizeof(cx<T>) * E));
     double *ang = reinterpret_cast<double *>(base + align16(sizeof(cx<T>) * E) + align16(sizeof(T) * E));
     cx<T> *cb = reinterpret_cast<cx<T> *>(reinterpret_cast<unsigned char *>(ang) + align16(sizeof(double) * E));
-    T *gA = reinterpret_cast<T *>(reinterpret_cast<unsigned char *>(cb) + align16(sizeof(cx<T>) * E));
-    T *gP = gA + nk;
     for (int p = blockIdx.x * wpb + wid; p < npath; p += gridDim.x * wpb) {
         for (int e = lane; e < E; e += FM_WAVE) cc[e] = mk<T>(cchi[((size_t)p * E + e) * 2], cchi[((size_t)p * E + e) * 2 + 1]);
         wave_sync();
         amp_phase<T>(E, cc, amp, ang);
-        const double rb = fine_grid_adjoint_q<T>(G.jq, G.fq, G.q, nk, reff[p], 0.0, amp, ang, nullptr, 0.0,
-                                                 gchi + (size_t)p * nk, gA, gP);
-        wave_sync();
-        fine_grid_gather<T>(E, G.qorder, G.qstart, G.fq, gA, gP, cc, amp, cb);
+        const double rb = fine_grid_adjoint<T>(E, G.jq, G.fq, G.q, G.qorder, G.qstart, reff[p], 0.0, nullptr, 0.0,
+                                               gchi + (size_t)p * nk, cc, amp, ang, cb);
         wave_sync();
         for (int e = lane; e < E; e += FM_WAVE) {
             gcchi[((size_t)p * E + e) * 2] = cb[e].r;
@@ -776,23 +904,38 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, i
     }
 }
 
+// Paths per workgroup: the largest plan within the LDS budget (by default one workgroup per CU, whose 4 wavefronts
+// then have the whole register file: 1 wavefront per SIMD); a plan small enough for two workgroups per CU runs the
+// variant sized for 2 wavefronts per SIMD (FP32).
 template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st) {
-    layout<T, SET>(a);
-    if (a.wave_bytes > 160 * 1024) return hipErrorInvalidValue;
-    int wpb = (int)((64 * 1024) / a.wave_bytes);
-    wpb = wpb < 1 ? 1 : (wpb > WPB_MAX ? WPB_MAX : wpb);
-    a.wpb = wpb;
-    const size_t shm = a.wave_bytes * wpb;
+    const fm_edge &ed = a.ed;
+    a.ltab = ed.L1 + 1;
+    layout<T, SET>(a, ed);
+    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(1) - 256);
+    if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+    if (g < 0) return hipErrorInvalidValue;
+    a.gp = 1 << g;
+    a.bp = ed.bands[g];
+    const size_t shm = a.path_bytes * a.gp;
+    const long slots = (long)(a.work_bytes / a.work_per_wave);
+    if (slots < a.bp.nband) return hipErrorInvalidValue;
+    long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
+    if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
+    if (blocks == 0) return hipSuccess;
+    const void *fn = (const void *)k_backward<T, SET, 1>;
+    if constexpr (sizeof(T) == 4)
+        if (shm <= FM_LDS_SHARE - 256) fn = (const void *)k_backward<T, SET, 2>;
     if (shm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_backward<T, SET>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const long slots = (long)(a.work_bytes / a.work_per_wave);
-    if (slots < wpb) return hipErrorInvalidValue;
-    long blocks = (a.npath_bucket + wpb - 1) / wpb;
-    if (blocks > slots / wpb) blocks = slots / wpb;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_backward<T, SET>), dim3((unsigned)blocks), dim3(FM_WAVE * wpb), shm, st, a);
+    if constexpr (sizeof(T) == 4) {
+        if (fn == (const void *)k_backward<T, SET, 2>) {
+            hipLaunchKernelGGL((k_backward<T, SET, 2>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_backward<T, SET, 1>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
@@ -806,18 +949,9 @@ template <class T> hipError_t dispatch_b(BArgs &a, int set, hipStream_t st) {
 }
 
 int check_edge_b(const fm_edge *e) {
-    if (!e) return set_err(FM_EINVAL, "edge table is NULL");
-    if (e->E < 2 || e->E > 64 * FM_MAX_ECHUNK) return set_err(FM_EINVAL, "edge: E out of range 2..256");
-    if (e->L1 < 1 || e->L1 > 120) return set_err(FM_EINVAL, "edge: L1 out of range 1..120");
-    if (e->nph1 < 1 || e->nph1 > FM_MAX_POT) return set_err(FM_EINVAL, "edge: nph1 out of range");
-    if (e->nchan < 1 || e->nchan > FM_MAX_CHAN) return set_err(FM_EINVAL, "edge: nchan out of range");
-    for (int c = 0; c < e->nchan; ++c)
-        if (e->chan_il[c] < 1 || e->chan_il[c] > 4 || e->chan_il[c] > e->L1)
-            return set_err(FM_EINVAL, "edge: channel il out of range 1..min(4, L1)");
-    for (int i = 0; i < e->nph1; ++i)
-        if (e->lmax1[i] < 1 || e->lmax1[i] > e->L1) return set_err(FM_EINVAL, "edge: lmax1 out of range");
+    const char *m = edge_problem(e);
+    if (m) return set_err(FM_EINVAL, m);
     if (e->ilinit < 1 || e->ilinit > 3) return set_err(FM_EUNSUPPORTED, "edge: only K, L and M edges (ilinit 1..3)");
-    if (!e->lmax || !e->ck || !e->ckmxk || !e->rotc || !e->rotn) return set_err(FM_EINVAL, "edge: NULL table");
     return FM_OK;
 }
 
@@ -830,12 +964,13 @@ int set_of(const fm_edge &ed, int nleg) {
 
 extern "C" {
 
-size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t wave_slots) {
-    if (nlegmax < 2 || nlegmax > FM_MAX_LEGS || wave_slots < 1) return 0;
-    const size_t es = precision == 32 ? 8 : 16;
+size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t band_slots) {
+    if (nlegmax < 2 || nlegmax > FM_MAX_LEGS || band_slots < 1) return 0;
+    const size_t ec = precision == 32 ? 8 : 16, er = ec / 2;
     const size_t nslots = nlegmax > 2 ? nlegmax - 1 : 1;
-    const size_t per = (es * nslots * 36 * FM_WAVE + 255) & ~size_t(255);
-    return per * (size_t)wave_slots;
+    const size_t cxe = nslots * 36 + (size_t)(nlegmax - 1) * FM_NREP + FM_UPARK;
+    const size_t per = (ec * cxe * FM_WAVE + er * (size_t)nacc(nlegmax) * FM_WAVE + 255) & ~size_t(255);
+    return per * (size_t)band_slots;
 }
 
 int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg,
@@ -849,14 +984,14 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     if (npath == 0) return FM_OK;
     if (!coords || !ipots || !gcchi || !gcoords || !workspace) return set_err(FM_EINVAL, "NULL buffer");
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
-    if (workspace_bytes < fm_backward_workspace_bytes(precision, nleg, WPB_MAX))
-        return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nleg, 4)");
+    if (workspace_bytes < fm_backward_workspace_bytes(precision, nleg, 16))
+        return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nleg, 16)");
     const int set = set_of(*edge, nleg);
     if (set < 0) return set_err(FM_EUNSUPPORTED, "two-leg paths of L and M edges are not implemented on the GPU");
     BArgs a;
     memset(&a, 0, sizeof a);
-    a.ed[0] = *edge;
-    a.nedge = 1;
+    a.ed = *edge;
+    a.edge_id = 0;
     a.npath_bucket = npath;
     a.src.mode = 0;
     a.src.nleg = nleg;
@@ -866,14 +1001,13 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     a.greff = greff;
     a.gcoord = gcoords;
     a.nslot = nleg;
-    a.ltab = edge->L1 + 1;
-    a.emax = edge->E;
     a.nk = 0;
     a.nlegmax = nleg;
     a.work = (unsigned char *)workspace;
     a.work_bytes = workspace_bytes;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
+    if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_cchi_backward: a path's tables do not fit one workgroup's LDS");
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi_backward launch");
 }
 
@@ -887,11 +1021,10 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
         !grid->qstart)
         return set_err(FM_EINVAL, "NULL buffer");
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
-    const int E = edge->E, nk = grid->nk;
+    const int E = edge->E;
     const size_t es = precision == 32 ? 4 : 8;
-    const size_t wb = align16(2 * es * E) + align16(es * E) + align16(8 * (size_t)E) + align16(2 * es * E) + align16(2 * es * nk);
-    if (wb > 64 * 1024) return set_err(FM_EINVAL, "grid too large for one workgroup's LDS");
-    const int wpb = (int)((64 * 1024) / wb) < WPB_MAX ? (int)((64 * 1024) / wb) : WPB_MAX;
+    const size_t wb = align16(2 * es * E) + align16(es * E) + align16(8 * (size_t)E) + align16(2 * es * E);
+    const int wpb = WPB_MAX;
     const int blocks = (npath + wpb - 1) / wpb;
     hipStream_t st = (hipStream_t)stream;
     if (precision == 32)
@@ -907,16 +1040,15 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
 int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                           int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
-                          const int32_t *bucket_nlegmax, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
+                          const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
                           void *workspace, size_t workspace_bytes, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
-    int emax = 0, ltab = 0;
+    int emax = 0;
     for (int i = 0; i < nedge; ++i) {
         int rc = check_edge_b(&edges[i]);
         if (rc) return rc;
         emax = emax > edges[i].E ? emax : edges[i].E;
-        ltab = ltab > edges[i].L1 + 1 ? ltab : edges[i].L1 + 1;
     }
     if (iorder != 2) return set_err(FM_EUNSUPPORTED, "only iorder = 2 is implemented on the GPU");
     if (!paths || !grid || !gcoord || !grid->qorder || !grid->qstart) return set_err(FM_EINVAL, "NULL paths, grid or gcoord");
@@ -925,19 +1057,21 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     if (paths->nmax < 2 || paths->nmax > FM_MAX_LEGS) return set_err(FM_EINVAL, "nmax out of range 2..9");
     if (grid->qstride < emax + 1) return set_err(FM_EINVAL, "grid.qstride < E + 1");
-    if (!workspace || workspace_bytes < fm_backward_workspace_bytes(precision, paths->nmax, WPB_MAX))
-        return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nmax, 4)");
+    if (!workspace || workspace_bytes < fm_backward_workspace_bytes(precision, paths->nmax, 16))
+        return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nmax, 16)");
+    if (nbucket > 0 && (!bucket_offset || !bucket_kind || !bucket_nleg)) return set_err(FM_EINVAL, "NULL bucket arrays");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {
         const int cnt = bucket_offset[b + 1] - bucket_offset[b];
         if (cnt <= 0) continue;
         const int set = bucket_kind[b];
         if (set < 0 || set > 2) return set_err(FM_EUNSUPPORTED, "bucket kind not implemented on the GPU");
-        if (bucket_edge && (bucket_edge[b] < 0 || bucket_edge[b] >= nedge)) return set_err(FM_EINVAL, "bucket edge out of range");
+        const int eid = bucket_edge ? bucket_edge[b] : 0;
+        if (eid < 0 || eid >= nedge) return set_err(FM_EINVAL, "bucket edge out of range");
         BArgs a;
         memset(&a, 0, sizeof a);
-        for (int i = 0; i < nedge; ++i) a.ed[i] = edges[i];
-        a.nedge = nedge;
+        a.ed = edges[eid];
+        a.edge_id = eid;
         a.npath_bucket = cnt;
         a.first = bucket_offset[b];
         a.order = order;
@@ -949,14 +1083,15 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         a.gpath = gpath;
         a.gcoord = gcoord;
         a.nslot = paths->nmax;
-        a.ltab = ltab;
-        a.emax = emax;
         a.nk = grid->nk;
-        int nlm = bucket_nlegmax ? bucket_nlegmax[b] : paths->nmax;
-        a.nlegmax = (nlm < 2 || nlm > paths->nmax) ? paths->nmax : nlm;
+        a.nlegmax = bucket_nleg[b];
+        if (a.nlegmax < 2 || a.nlegmax > paths->nmax) return set_err(FM_EINVAL, "bucket nleg out of range 2..nmax");
+        if ((set == 2) != (a.nlegmax == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
         a.work = (unsigned char *)workspace;
         a.work_bytes = workspace_bytes;
         hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
+        if (e == hipErrorInvalidValue)
+            return set_err(FM_EINVAL, "fm_chi_paths_backward: the bucket's paths do not fit one workgroup's LDS");
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths_backward launch");
     }
     if (grad_pos && natom > 0) {

@@ -370,13 +370,13 @@ __device__ __forceinline__ Dd arg_dual(Dd x, Dd y, Dd fi) {
     return (x.v == 0.0 && y.v == 0.0) ? fi : datan2(y, x);
 }
 struct LegGeoD { Dd r, beta, alpha, gamma; };
-__device__ __forceinline__ LegGeoD leg_geometry_dual(const Dd (*rat)[3], int J, int n) {
-    int i, ip1, im1;
-    if (J == n) { i = 0; ip1 = 1; im1 = n - 1; } else { i = J; ip1 = J + 1; im1 = J - 1; }
+// The leg from atom `pm` to atom `pi`, the next leg from `pi` to `pp` (leg_geometry's im1, i, ip1).
+__device__ __forceinline__ LegGeoD leg_geometry_dual(const Dd (&pm)[3], const Dd (&pi)[3], const Dd (&pp)[3]) {
     Dd vo[3], vi[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        vo[c] = rat[ip1][c] - rat[i][c];
-        vi[c] = rat[i][c] - rat[im1][c];
+        vo[c] = pp[c] - pi[c];
+        vi[c] = pi[c] - pm[c];
     }
     TrigD p = trig_dual(vo[0], vo[1], vo[2]);
     TrigD q = trig_dual(vi[0], vi[1], vi[2]);

@@ -1,11 +1,15 @@
 // Fused path-stage forward kernels for gfx950 and the C ABI that launches them.
 //
-// One wavefront per path, one lane per phase-grid energy (energies beyond 64 loop). Per path the wave computes
-// the leg geometry in FP64 (one lane per leg), the rotation tables d^l_{m1 m2}(beta_s) of every leg into LDS (one
-// lane per (leg, pair)), then every lane evaluates the Rehr-Albers amplitude at its energy: the termination matrix
-// T, then the scattering matrices F_{n-1} ... F_1 multiplied into U = T F_{n-1} ... F_s, each F streamed over the
-// partial waves with the spherical-wave recursion of its two legs kept in registers; cchi = tr(U) cfac. The fine
-// grid (amplitude and unwrapped phase interpolated to q, chi = A sin(Phi + 2 q reff)) is the wave's epilogue.
+// A workgroup of 4 wavefronts holds G paths (1, 2, 4 or 8: as many as fit the LDS) of the same edge and leg count.
+// Per path one wavefront computes the leg geometry in FP64 (one lane per leg) and the rotation tables
+// d^l_{m1 m2}(beta_s) of every leg into LDS (one lane per (leg, pair)). The (path, energy) pairs are then cut into
+// energy bands of the edge's plan: a wavefront runs whole bands, lane (path j, energy i) = j * (64 / G) + i, so its
+// lanes loop over similar numbers of partial waves, and the bands are spread over the wavefronts so that their
+// loads balance. Every lane evaluates the Rehr-Albers amplitude at its energy: the termination matrix T, then the
+// scattering matrices F_{n-1} ... F_1 multiplied into U = T F_{n-1} ... F_s, each F streamed over the lane's partial
+// waves with the spherical-wave recursion of its two legs kept in registers; cchi = tr(U) cfac. The fine grid
+// (amplitude and unwrapped phase interpolated to q, chi = A sin(Phi + 2 q reff)) is one wavefront's epilogue per
+// path.
 //
 // Reference semantics: GPUPathStage.cchi / chi_fine (src/feffgpu/stage.py:235-388), the literal port
 // genfmt.PathStage (src/feffgpu/genfmt.py:395-445) and the path-list gather/sum of
@@ -14,6 +18,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <type_traits>
 
 #include "fm_device.h"
@@ -27,11 +32,16 @@ int fm_set_err(int code, const char *msg) {
     return code;
 }
 
+// LDS a workgroup may take when the paths per workgroup are chosen: forward, adjoint (bytes).
+static std::atomic<size_t> g_lds_budget[2] = {{80 * 1024}, {160 * 1024}};
+size_t fm_lds_budget(int kernel) { return g_lds_budget[kernel ? 1 : 0].load(std::memory_order_relaxed); }
+
 namespace {
 
 struct KArgs {
-    fm_edge ed[FM_MAX_EDGE];
-    int nedge;
+    fm_edge ed;           // the edge of every path of this launch
+    fm_bands bp;          // its band plan for gp paths per workgroup
+    int gp;
     int npath_bucket;     // paths in this launch
     int first;            // offset into order[]
     const int32_t *order;
@@ -40,9 +50,8 @@ struct KArgs {
     fm_grid G;            // mode 1
     void *chi_path;
     double *reff_out;
-    int ltab;             // L1 + 1 rows per rotation table
-    int emax;             // largest E over the edges
-    int edge_id;          // the edge of every path of this launch
+    int edge_id;          // index of the edge (rows of the grid's jq / fq)
+    int nleg;             // the leg count of every path of this launch
     size_t path_bytes;    // LDS per path of a workgroup
 };
 
@@ -56,78 +65,76 @@ int hip_err(hipError_t e, const char *where) {
 }
 
 
-// A workgroup holds G = ed.gpaths paths and 4 wavefronts. Setup: wave j < G loads path j (geometry, rotation tables)
-// into its LDS area. Energies: wave w takes the energy band [gstart[w], gstart[w+1]) of all G paths, lane
-// (path j, energy i) = j * (64 / G) + i: the lanes of a wave share similar partial-wave counts. Epilogue: wave j
-// interpolates path j to the output grid.
 template <class T, int SET>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_WPE(T)))) void k_forward(KArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int wid = threadIdx.x / FM_WAVE;
+    __shared__ int pidx[8], pval[8];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
     const int lane = lane_id();
-    const fm_edge &ed = a.ed[a.edge_id];
-    const int G = ed.gpaths, cap = FM_WAVE / G;
+    const fm_edge &ed = a.ed;
+    const int G = a.gp, cap = FM_WAVE / G;
     const int ltab = ed.L1 + 1, E = ed.E;
+    const int nl = a.nleg;
     const size_t off_W = align16(sizeof(PathLDS));
-    const int nl = a.src.mode == 0 ? a.src.nleg : a.src.P.nmax;     // rotation tables sized for the longest path
-    const size_t off_amp = off_W + align16(sizeof(T) * (size_t)nl * ltab * FM_NREP);
+    const size_t off_amp = off_W + align16(sizeof(T) * rot_elems(nl, ltab));
     const size_t off_ang = off_amp + align16(sizeof(T) * (size_t)E);
-    __shared__ int pidx[4];
-    __shared__ int pval[4];
+    const int jl = lane / cap, il = lane % cap;      // this lane's path and energy within a band
+    unsigned char *lbase = smem + (size_t)jl * a.path_bytes;
+    const PathLDS &gl = *reinterpret_cast<const PathLDS *>(lbase);
+    const T *Wl = reinterpret_cast<const T *>(lbase + off_W);
     for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
-        // ---- setup: one wave per path
-        if (wid < G) {
-            const int slot = blk * G + wid;
+        // ---- setup: one wavefront per path
+        for (int j = wid; j < G; j += 4) {
+            const int slot = blk * G + j;
             const bool valid = slot < a.npath_bucket;
             const int s0 = valid ? slot : blk * G;
             const int p = a.order ? a.order[a.first + s0] : a.first + s0;
-            unsigned char *base = smem + (size_t)wid * a.path_bytes;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
             PathLDS &g = *reinterpret_cast<PathLDS *>(base);
             int ei;
-            const int n = load_path(a.src, p, g, ei);
-            build_rotation<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + off_W));
-            if (lane == 0) { pidx[wid] = p; pval[wid] = valid ? 1 : 0; }
+            load_path(a.src, p, g, ei, nl);
+            build_rotation<T, SET>(ed, g, nl, ltab, reinterpret_cast<T *>(base + off_W));
+            if (lane == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
         }
         __syncthreads();
-        // ---- energies: band wid of every path
-        {
-            const int e0 = ed.gstart[wid], nb = ed.gstart[wid + 1] - e0;
-            const int j = lane / cap, i = lane % cap;
-            const bool active = nb > 0 && i < nb && j < G;
-            const int jj = j < G ? j : G - 1;
-            const int e = e0 + (i < nb ? i : (nb > 0 ? nb - 1 : 0));
-            unsigned char *base = smem + (size_t)jj * a.path_bytes;
-            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
-            const T *W = reinterpret_cast<const T *>(base + off_W);
-            const int ee = e < E ? e : E - 1;
-            const cx<T> c = path_cchi<T, SET>(ed, g, W, ltab, ee, g.n);
-            if (active && pval[jj]) {
+        // ---- energies: the bands of this wavefront
+        for (int b = 0; b < a.bp.nband; ++b) {
+            if (a.bp.wave[b] != wid) continue;
+            const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
+            const bool active = il < nb && pval[jl];
+            const int e = e0 + (il < nb ? il : nb - 1);
+            const cx<T> c = path_cchi<T, SET>(ed, gl, Wl, ltab, e, nl);
+            if (active) {
                 if (a.src.mode == 0) {
-                    T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)pidx[jj] * E + e) * 2;
+                    T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)pidx[jl] * E + e) * 2;
                     o[0] = c.r;
                     o[1] = c.i;
                 } else {
-                    T *amp = reinterpret_cast<T *>(base + off_amp);
-                    double *ang = reinterpret_cast<double *>(base + off_ang);
+                    T *amp = reinterpret_cast<T *>(lbase + off_amp);
+                    double *ang = reinterpret_cast<double *>(lbase + off_ang);
                     amp[e] = (T)sqrt((double)c.r * c.r + (double)c.i * c.i + 1e-30);
                     ang[e] = (double)atan2(c.i, c.r);
                 }
             }
         }
         __syncthreads();
-        // ---- epilogue: one wave per path
-        if (wid < G && pval[wid]) {
-            unsigned char *base = smem + (size_t)wid * a.path_bytes;
+        // ---- epilogue: one wavefront per path
+        for (int j = wid; j < G; j += 4) {
+            if (!pval[j]) continue;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
             const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
-            const int p = pidx[wid];
-            const double reff_A = g.reff * FM_BOHR;
+            const int p = pidx[j];
+            const double reff_A = g.ok ? g.reff * FM_BOHR : __builtin_nan("");
             if (lane == 0 && a.reff_out) a.reff_out[p] = reff_A;
             if (a.src.mode == 1) {
                 const int nk = a.G.nk;
                 T *out = reinterpret_cast<T *>(a.chi_path) + (size_t)p * nk;
-                fine_grid<T>(ed, a.G.jq + (size_t)a.edge_id * nk, a.G.fq + (size_t)a.edge_id * nk, a.G.q, nk, reff_A,
-                             a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, reinterpret_cast<T *>(base + off_amp),
-                             reinterpret_cast<double *>(base + off_ang), out);
+                if (g.ok)
+                    fine_grid<T>(ed, a.G.jq + (size_t)a.edge_id * nk, a.G.fq + (size_t)a.edge_id * nk, a.G.q, nk, reff_A,
+                                 a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, reinterpret_cast<T *>(base + off_amp),
+                                 reinterpret_cast<double *>(base + off_ang), out);
+                else
+                    for (int i = lane; i < nk; i += FM_WAVE) out[i] = (T)__builtin_nan("");
             }
         }
         __syncthreads();
@@ -156,87 +163,47 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine(fm_edge ed, int n
     }
 }
 
-// chi_seg[s][k] = sum over the CSR list of segment s of deg[p] chi_path[p][k]: a workgroup per (64 k-points,
-// segment); wave w sums the list entries w, w + 8, ... in order, then the 8 partial sums are added in wave order.
-// Deterministic for a given list and grid.
-constexpr int SEG_WAVES = 8;
-template <class T>
-__global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, int nk, const int32_t *seg_ptr,
-                                                                     const int32_t *seg_paths, const double *deg,
-                                                                     const T *chi_path, double *chi_seg) {
-    __shared__ double part[SEG_WAVES][FM_WAVE];
-    const int s = blockIdx.y;
-    const int w = threadIdx.x / FM_WAVE, lane = threadIdx.x % FM_WAVE;
-    const int k = blockIdx.x * FM_WAVE + lane;
-    double acc = 0.0;
-    if (k < nk) {
-        for (int t = seg_ptr[s] + w; t < seg_ptr[s + 1]; t += SEG_WAVES) {
-            const int p = seg_paths[t];
-            acc += deg[p] * (double)chi_path[(size_t)p * nk + k];
-        }
-    }
-    part[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && k < nk) {
-        double tot = 0.0;
-        for (int i = 0; i < SEG_WAVES; ++i) tot += part[i][lane];
-        chi_seg[(size_t)s * nk + k] = tot;
-    }
-}
-
 // ------------------------------------------------------------------------------------------------ launch helpers
 int set_for(const fm_edge &ed, int nleg) {
     if (ed.ilinit <= 1) return nleg == 2 ? 2 : 0;
     return nleg == 2 ? -1 : 1;
 }
 
-template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st, int nlegmax) {
-    const fm_edge &ed = a.ed[a.edge_id];
-    a.ltab = ed.L1 + 1;
-    a.emax = ed.E;
-    const int nl = a.src.mode == 0 ? a.src.nleg : a.src.P.nmax;
-    (void)nlegmax;
-    a.path_bytes = align16(sizeof(PathLDS)) + align16(sizeof(T) * (size_t)nl * a.ltab * FM_NREP) + align16(sizeof(T) * ed.E) +
+// Paths per workgroup: the largest plan within the LDS budget (by default room for two workgroups per CU), else the
+// largest that fits one workgroup.
+template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st) {
+    const fm_edge &ed = a.ed;
+    a.path_bytes = align16(sizeof(PathLDS)) + align16(sizeof(T) * rot_elems(a.nleg, ed.L1 + 1)) + align16(sizeof(T) * ed.E) +
                    align16(sizeof(double) * ed.E);
-    const size_t shm = a.path_bytes * ed.gpaths;
-    if (shm > 160 * 1024) return hipErrorInvalidValue;
+    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(0) - 256);
+    if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+    if (g < 0) return hipErrorInvalidValue;
+    a.gp = 1 << g;
+    a.bp = ed.bands[g];
+    const size_t shm = a.path_bytes * a.gp;
     if (shm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_forward<T, SET>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const int blocks = (a.npath_bucket + ed.gpaths - 1) / ed.gpaths;
+    const int blocks = (a.npath_bucket + a.gp - 1) / a.gp;
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL((k_forward<T, SET>), dim3(blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
-template <class T> hipError_t dispatch(KArgs &a, int set, hipStream_t st, int nlegmax) {
+template <class T> hipError_t dispatch(KArgs &a, int set, hipStream_t st) {
     switch (set) {
-        case 0: return launch_forward<T, 0>(a, st, nlegmax);
-        case 1: return launch_forward<T, 1>(a, st, nlegmax);
-        case 2: return launch_forward<T, 2>(a, st, nlegmax);
+        case 0: return launch_forward<T, 0>(a, st);
+        case 1: return launch_forward<T, 1>(a, st);
+        case 2: return launch_forward<T, 2>(a, st);
     }
     return hipErrorInvalidValue;
 }
 
 int check_edge(const fm_edge *e) {
-    if (!e) return set_err(FM_EINVAL, "edge table is NULL");
-    if (e->E < 2 || e->E > 64 * FM_MAX_ECHUNK) return set_err(FM_EINVAL, "edge: E out of range 2..256");
-    if (e->L1 < 1 || e->L1 > 120) return set_err(FM_EINVAL, "edge: L1 out of range 1..120");
-    if (e->nph1 < 1 || e->nph1 > FM_MAX_POT) return set_err(FM_EINVAL, "edge: nph1 out of range");
-    if (e->nchan < 1 || e->nchan > FM_MAX_CHAN) return set_err(FM_EINVAL, "edge: nchan out of range");
-    for (int c = 0; c < e->nchan; ++c)
-        if (e->chan_il[c] < 1 || e->chan_il[c] > 4 || e->chan_il[c] > e->L1)
-            return set_err(FM_EINVAL, "edge: channel il out of range 1..min(4, L1)");
-    for (int i = 0; i < e->nph1; ++i)
-        if (e->lmax1[i] < 1 || e->lmax1[i] > e->L1) return set_err(FM_EINVAL, "edge: lmax1 out of range");
+    const char *m = edge_problem(e);
+    if (m) return set_err(FM_EINVAL, m);
     if (e->ilinit < 1 || e->ilinit > 3) return set_err(FM_EUNSUPPORTED, "edge: only K, L and M edges (ilinit 1..3)");
-    if (!e->lmax || !e->ck || !e->ckmxk || !e->rotc || !e->rotn) return set_err(FM_EINVAL, "edge: NULL table");
-    if (e->gpaths != 1 && e->gpaths != 2 && e->gpaths != 4) return set_err(FM_EINVAL, "edge: gpaths must be 1, 2 or 4");
-    if (e->gstart[0] != 0 || e->gstart[4] != e->E) return set_err(FM_EINVAL, "edge: energy bands must cover 0..E");
-    for (int w = 0; w < 4; ++w)
-        if (e->gstart[w + 1] < e->gstart[w] || e->gstart[w + 1] - e->gstart[w] > FM_WAVE / e->gpaths)
-            return set_err(FM_EINVAL, "edge: an energy band is longer than 64 / gpaths");
     return FM_OK;
 }
 
@@ -246,6 +213,11 @@ extern "C" {
 
 const char *fm_last_error(void) { return g_err; }
 const char *fm_version(void) { return "feffgpu_amd 0.1 (gfx950)"; }
+
+int32_t fm_set_lds_budget(int32_t kernel, int32_t kib) {
+    if ((kernel != 0 && kernel != 1) || kib < 16 || kib > 160) return -1;
+    return (int32_t)(g_lds_budget[kernel].exchange((size_t)kib * 1024) / 1024);
+}
 
 int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg, const double *coords,
             const int64_t *ipots, void *cchi, double *reff, void *stream) {
@@ -261,8 +233,7 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
     if (set < 0) return set_err(FM_EUNSUPPORTED, "two-leg paths of L and M edges are not implemented on the GPU");
     KArgs a;
     memset(&a, 0, sizeof a);
-    a.ed[0] = *edge;
-    a.nedge = 1;
+    a.ed = *edge;
     a.src.mode = 0;
     a.npath_bucket = npath;
     a.first = 0;
@@ -273,8 +244,10 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
     a.cchi_out = cchi;
     a.reff_out = reff;
     a.edge_id = 0;
+    a.nleg = nleg;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = precision == 32 ? dispatch<float>(a, set, st, nleg) : dispatch<double>(a, set, st, nleg);
+    hipError_t e = precision == 32 ? dispatch<float>(a, set, st) : dispatch<double>(a, set, st);
+    if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_cchi: a path's tables do not fit one workgroup's LDS");
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi launch");
 }
 
@@ -303,7 +276,7 @@ int fm_chi_fine(const fm_edge *edge, int32_t precision, int32_t npath, const voi
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                          int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
                          const int32_t *bucket_offset, const int32_t *bucket_set, const int32_t *bucket_edge,
-                         const int32_t *bucket_nlegmax, int32_t nseg, const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg,
+                         const int32_t *bucket_nleg, int32_t nseg, const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg,
                          void *chi_path, double *reff, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     for (int i = 0; i < nedge; ++i) {
@@ -315,6 +288,7 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     if (paths->nmax < 2 || paths->nmax > FM_MAX_LEGS) return set_err(FM_EINVAL, "nmax out of range 2..9");
     if (grid->nk < 1) return set_err(FM_EINVAL, "empty grid");
+    if (nbucket > 0 && (!bucket_offset || !bucket_set || !bucket_nleg)) return set_err(FM_EINVAL, "NULL bucket arrays");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {
         const int cnt = bucket_offset[b + 1] - bucket_offset[b];
@@ -323,8 +297,6 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         if (set < 0 || set > 2) return set_err(FM_EUNSUPPORTED, "bucket kind not implemented on the GPU");
         KArgs a;
         memset(&a, 0, sizeof a);
-        for (int i = 0; i < nedge; ++i) a.ed[i] = edges[i];
-        a.nedge = nedge;
         a.src.mode = 1;
         a.npath_bucket = cnt;
         a.first = bucket_offset[b];
@@ -335,9 +307,12 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         a.reff_out = reff;
         a.edge_id = bucket_edge ? bucket_edge[b] : 0;
         if (a.edge_id < 0 || a.edge_id >= nedge) return set_err(FM_EINVAL, "bucket edge out of range");
-        int nlm = bucket_nlegmax ? bucket_nlegmax[b] : paths->nmax;
-        if (nlm < 2 || nlm > paths->nmax) nlm = paths->nmax;
-        hipError_t e = precision == 32 ? dispatch<float>(a, set, st, nlm) : dispatch<double>(a, set, st, nlm);
+        a.ed = edges[a.edge_id];
+        a.nleg = bucket_nleg[b];
+        if (a.nleg < 2 || a.nleg > paths->nmax) return set_err(FM_EINVAL, "bucket nleg out of range 2..nmax");
+        if ((set == 2) != (a.nleg == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
+        hipError_t e = precision == 32 ? dispatch<float>(a, set, st) : dispatch<double>(a, set, st);
+        if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_chi_paths: the bucket's paths do not fit one workgroup's LDS");
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths launch");
     }
     if (chi_seg && nseg > 0) {

@@ -32,24 +32,92 @@ struct PathLDS {
     double c1[FM_MAX_LEGS + 1], s1[FM_MAX_LEGS + 1], c2[FM_MAX_LEGS + 1], s2[FM_MAX_LEGS + 1];  // cos/sin eta, 2 eta
     int ipot[FM_MAX_LEGS + 1];       // potential of path atom j (0 and n: absorber)
     double reff, R, prod_r;           // bohr
-    int n, pad_[3];
+    int n, ok, pad_[2];               // ok = 0: placeholder for a path whose leg count is not the launch's
 };
 
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-template <class T> __host__ __device__ inline size_t wave_lds_bytes(int nlegmax, int ltab, int emax) {
-    return align16(sizeof(PathLDS)) + align16(sizeof(T) * (size_t)nlegmax * ltab * FM_NREP) +
-           align16(sizeof(T) * (size_t)emax) + align16(sizeof(double) * (size_t)emax);
+// Elements of the rotation tables of a path of at most nl legs: legs 1..nl-1 hold ltab rows each; the last leg
+// (the termination's) is read at rows 1..4 only, and is stored with 5 rows at leg offset n - 1 <= nl - 1.
+__host__ __device__ inline size_t rot_elems(int nl, int ltab) { return ((size_t)(nl - 1) * ltab + 5) * FM_NREP; }
+
+// ------------------------------------------------------------------------------------------------ host checks
+constexpr int FM_MAX_E = 256;
+constexpr size_t FM_LDS_MAX = 160 * 1024;       // LDS of one CU
+constexpr size_t FM_LDS_SHARE = 80 * 1024;      // what lets two workgroups share a CU
+
+// The edge's tables and band plans are consistent; returns the problem or nullptr.
+inline const char *edge_problem(const fm_edge *e) {
+    if (!e) return "edge table is NULL";
+    if (e->E < 2 || e->E > FM_MAX_E) return "edge: E out of range 2..256";
+    if (e->L1 < 1 || e->L1 > 120) return "edge: L1 out of range 1..120";
+    if (e->nph1 < 1 || e->nph1 > FM_MAX_POT) return "edge: nph1 out of range";
+    if (e->nchan < 1 || e->nchan > FM_MAX_CHAN) return "edge: nchan out of range";
+    for (int c = 0; c < e->nchan; ++c)
+        if (e->chan_il[c] < 1 || e->chan_il[c] > 4 || e->chan_il[c] > e->L1) return "edge: channel il out of range 1..min(4, L1)";
+    for (int i = 0; i < e->nph1; ++i)
+        if (e->lmax1[i] < 1 || e->lmax1[i] > e->L1) return "edge: lmax1 out of range";
+    if (!e->lmax || !e->ck || !e->ckmxk || !e->rotc || !e->rotn) return "edge: NULL table";
+    for (int g = 0; g < 4; ++g) {
+        const fm_bands &b = e->bands[g];
+        if (b.nband == 0) continue;
+        if (b.nband < 0 || b.nband > 16) return "edge: band count out of range 0..16";
+        if (b.start[0] != 0 || b.start[b.nband] != e->E) return "edge: bands must cover the energies 0..E";
+        for (int k = 0; k < b.nband; ++k) {
+            const int len = b.start[k + 1] - b.start[k];
+            if (len < 1 || len > FM_WAVE >> g) return "edge: a band holds 0 or more than 64 / gpaths energies";
+            if (b.wave[k] < 0 || b.wave[k] > 3) return "edge: band wavefront out of range 0..3";
+        }
+    }
+    if (e->bands[0].nband == 0) return "edge: no band plan for one path per workgroup";
+    return nullptr;
+}
+
+// log2 of the largest gpaths with a band plan whose paths (path_bytes of LDS each) fit `budget`; -1 if none.
+inline int pick_gpaths(const fm_edge &e, size_t path_bytes, size_t budget) {
+    for (int g = 3; g >= 0; --g)
+        if (e.bands[g].nband > 0 && (path_bytes << g) <= budget) return g;
+    return -1;
 }
 
 // ------------------------------------------------------------------------------------------------ setup of one path
-// Fills g (geometry, potentials) for path slot `p`; all lanes call it. Returns n.
-__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx) {
+// Point j of path p of a list, in bohr relative to its absorber (point 0 = the absorber itself), by the minimum
+// image of the list's cell (src/feffgpu/exafs/forward.py:153-156, src/feffgpu/cell.py:26-35).
+__device__ inline void path_point(const fm_paths &P, int p, int j, double (&v)[3]) {
+    v[0] = v[1] = v[2] = 0.0;
+    if (j > 0) {
+        const int at = P.atoms[(size_t)p * (P.nmax - 1) + j - 1];
+        const int ab = P.absorber[p];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = P.pos[3 * (size_t)at + c] - P.pos[3 * (size_t)ab + c];
+        if (P.cell_kind == 1) {
+            const double L = P.cell[0];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] -= L * rint(v[c] / L);
+        } else if (P.cell_kind == 2) {
+            double f[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] = v[0] * P.cellinv[0 * 3 + c] + v[1] * P.cellinv[1 * 3 + c] + v[2] * P.cellinv[2 * 3 + c];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] -= rint(f[c]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = f[0] * P.cell[0 * 3 + c] + f[1] * P.cell[1 * 3 + c] + f[2] * P.cell[2 * 3 + c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] /= FM_BOHR;
+}
+
+// Fills g (geometry, potentials) for path slot `p`; all lanes call it. Every path of a launch has nuni legs (the
+// kernels loop over the legs uniformly): a path-list entry with another leg count is replaced by a fixed
+// placeholder path of nuni legs with g.ok = 0, so that every index stays in bounds; the kernels write NaN for it.
+__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni) {
     const int lane = lane_id();
     int n;
+    bool ok = true;
     if (a.mode == 0) {
-        n = a.nleg;
+        n = nuni;
         edge_idx = 0;
         if (lane < n) {
             const double *c = a.coords + ((size_t)p * n + lane) * 3;
@@ -59,40 +127,28 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
             g.ipot[lane] = (int)a.ipots64[(size_t)p * n + lane];
         }
     } else {
-        n = a.P.nleg[p];
+        ok = a.P.nleg[p] == nuni;
+        n = nuni;
         edge_idx = a.P.edge ? a.P.edge[p] : 0;
         const int nmax = a.P.nmax;
-        if (lane < n) {
-            double v[3] = {0.0, 0.0, 0.0};
-            if (lane > 0) {
-                const int at = a.P.atoms[(size_t)p * (nmax - 1) + lane - 1];
-                const int ab = a.P.absorber[p];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = a.P.pos[3 * (size_t)at + c] - a.P.pos[3 * (size_t)ab + c];
-                if (a.P.cell_kind == 1) {
-                    const double L = a.P.cell[0];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[c] -= L * rint(v[c] / L);
-                } else if (a.P.cell_kind == 2) {
-                    double f[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        f[c] = v[0] * a.P.cellinv[0 * 3 + c] + v[1] * a.P.cellinv[1 * 3 + c] + v[2] * a.P.cellinv[2 * 3 + c];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) f[c] -= rint(f[c]);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        v[c] = f[0] * a.P.cell[0 * 3 + c] + f[1] * a.P.cell[1 * 3 + c] + f[2] * a.P.cell[2 * 3 + c];
-                }
+        if (!ok) {
+            if (lane < n) {
+                g.rat[lane][0] = 4.0 * lane;
+                g.rat[lane][1] = 1.5 * (lane & 1);
+                g.rat[lane][2] = 0.5 * lane;
+                g.ipot[lane] = 0;
             }
+        } else if (lane < n) {
+            double v[3];
+            path_point(a.P, p, lane, v);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) g.rat[lane][c] = v[c] / FM_BOHR;
+            for (int c = 0; c < 3; ++c) g.rat[lane][c] = v[c];
             g.ipot[lane] = a.P.ipots[(size_t)p * nmax + lane];
         }
     }
     wave_sync();
     if (lane < 3) g.rat[n][lane] = g.rat[0][lane];
-    if (lane == 0) { g.ipot[n] = g.ipot[0]; g.n = n; }
+    if (lane == 0) { g.ipot[n] = g.ipot[0]; g.n = n; g.ok = ok ? 1 : 0; }
     wave_sync();
     // one lane per leg
     if (lane < n) {
@@ -123,8 +179,9 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
 }
 
 // Rotation tables: W[s][il][k] = d^{il-1}_{m1 m2}(beta_{s+1}) xnlm(il,|m1|)(-1)^|m1| / xnlm(il,|m2|) for legs
-// s = 0..n-1 and the FM_NREP representatives k, il = 1..L1 (row 0 unused). Three-term recursion in l for fixed
-// (m1, m2) (src/feffgpu/stage.py:77-92), seeded by the closed forms at l <= 2, in FP64.
+// s = 0..n-1 and the FM_NREP representatives k, il = 1..L1 (row 0 unused; the last leg, which only the
+// termination reads, up to il = 4). Three-term recursion in l for fixed (m1, m2) (src/feffgpu/stage.py:77-92),
+// seeded by the closed forms at l <= 2, in FP64.
 template <class T, int SET>
 __device__ void build_rotation(const fm_edge &ed, const PathLDS &g, int n, int ltab, T *W) {
     constexpr PairDef PD = pairdef(SET);
@@ -141,8 +198,9 @@ __device__ void build_rotation(const fm_edge &ed, const PathLDS &g, int n, int l
         T *Ws = W + (size_t)s * ltab * FM_NREP;
         const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
         const int mm = (m1 + 3) * 7 + (m2 + 3), nn = am1 * 4 + am2;
+        const int rows = s == n - 1 ? (ed.L1 < 4 ? ed.L1 : 4) : ed.L1;
         double dlm1 = 0.0, dl = 0.0;
-        for (int il = 1; il <= ed.L1; ++il) {
+        for (int il = 1; il <= rows; ++il) {
             const int l = il - 1;
             double d;
             if (l <= 2) {
@@ -178,10 +236,10 @@ __device__ __forceinline__ void accum_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&xa
 
 // F_s of the path at the lane's energy: X = Hankel factors of leg s+1 (gam), Y = of leg s (gamtl) times the
 // scatterer's t-matrix, summed over il <= ilx with the rotation table of leg s (src/feffgpu/stage.py:287-305).
-// Accumulates the representatives k with ri < NR, rj < NC into Ft (zeroed here). Rows up to ilmax (the wave's
-// largest ilx) are generated; a lane adds only its rows il <= ilx.
+// Accumulates the representatives k with ri < NR, rj < NC into Ft (zeroed here). Each lane runs its own rows
+// il = 1..ilx (the wave runs to the largest ilx of its lanes); the t-matrix of the next row is fetched one row ahead.
 template <class T, int SET, int NR, int NC>
-__device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T rY, int ilx, int ilmax,
+__device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
                                        const T *__restrict__ Wl, const T *__restrict__ tl, int tlstride) {
 #pragma unroll
     for (int k = 0; k < FM_NREP; ++k) Ft[k] = mk<T>(T(0), T(0));
@@ -190,20 +248,24 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
     X.init(minus_i_over(ck * rX));
     Y.init(minus_i_over(ck * rY));
     cx<T> xa[3], yb[2];
-    const int rseed = ilmax < 4 ? ilmax : 4;
+    cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
+    const int rseed = ilx < 4 ? ilx : 4;
     for (int r = 1; r <= rseed; ++r) {
         X.seed_row(r, xa);
         if (r < 4) Y.seed_row(r, yb); else Y.step(r, yb);
-        const cx<T> t = mk<T>(tl[(size_t)r * tlstride], tl[(size_t)r * tlstride + 1]);
-        if (r <= ilx) accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
+        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
+        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
+        t = tn;
     }
 #pragma unroll 1
-    for (int r = 5; r <= ilmax; ++r) {
+    for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa);
         Y.step(r, yb);
-        const cx<T> t = mk<T>(tl[(size_t)r * tlstride], tl[(size_t)r * tlstride + 1]);
-        const T *w = Wl + (size_t)r * FM_NREP;
-        if (r <= ilx) accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, w);
+        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
+        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
+        t = tn;
     }
 }
 
@@ -285,10 +347,9 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
     cx<T> ptrac = mk<T>(T(0), T(0));
     cx<T> Ft[FM_NREP];
     cx<T> ph[6];
-    auto scat = [&](int s, int &ilx, int &ilmax, const T *&tl, const T *&Wl) {
+    auto scat = [&](int s, int &ilx, const T *&tl, const T *&Wl) {
         const int ip = g.ipot[s];
         ilx = ed.lmax[(size_t)ip * E + ee] + 1;
-        ilmax = ed.lmax1[ip];
         tl = tlbase + ((size_t)ip * ltab) * tlstride + 2 * ee;
         Wl = W + (size_t)(s - 1) * ltab * FM_NREP;
     };
@@ -314,10 +375,10 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
         }
     };
     if (n == 2) {
-        int ilx, ilmax;
+        int ilx;
         const T *tl, *Wl;
-        scat(1, ilx, ilmax, tl, Wl);
-        fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, ilmax, Wl, tl, tlstride);
+        scat(1, ilx, tl, Wl);
+        fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
         row_phases<T, SET>(ph, g, 2);
 #pragma unroll
         for (int a = 0; a < L0; ++a)
@@ -329,28 +390,28 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
     } else {
         // s = n - 1: F is L0 x L; U (L0 x L0) -> U F (L0 x L)
         {
-            int ilx, ilmax;
+            int ilx;
             const T *tl, *Wl;
-            scat(n - 1, ilx, ilmax, tl, Wl);
-            fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, ilmax, Wl, tl, tlstride);
+            scat(n - 1, ilx, tl, Wl);
+            fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
             row_phases<T, SET>(ph, g, n);
             chain(std::integral_constant<int, L0>{});
         }
 #pragma unroll 1
         for (int s = n - 2; s >= 2; --s) {
-            int ilx, ilmax;
+            int ilx;
             const T *tl, *Wl;
-            scat(s, ilx, ilmax, tl, Wl);
-            fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, ilmax, Wl, tl, tlstride);
+            scat(s, ilx, tl, Wl);
+            fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
             row_phases<T, SET>(ph, g, s + 1);
             chain(std::integral_constant<int, L>{});
         }
         // s = 1: F is L x L0; trace of U (L0 x L) F (L x L0)
         {
-            int ilx, ilmax;
+            int ilx;
             const T *tl, *Wl;
-            scat(1, ilx, ilmax, tl, Wl);
-            fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, ilmax, Wl, tl, tlstride);
+            scat(1, ilx, tl, Wl);
+            fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
             row_phases<T, SET>(ph, g, 2);
 #pragma unroll
             for (int a = 0; a < L0; ++a)
@@ -425,5 +486,33 @@ __device__ void fine_grid(const fm_edge &ed, const int32_t *jq, const double *fq
     }
 }
 
+
+// chi_seg[s][k] = sum over the CSR list of segment s of deg[p] chi_path[p][k]: a workgroup per (64 k-points,
+// segment); wave w sums the list entries w, w + 8, ... in order, then the 8 partial sums are added in wave order.
+// Deterministic for a given list and grid.
+constexpr int SEG_WAVES = 8;
+template <class T>
+__global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, int nk, const int32_t *seg_ptr,
+                                                                     const int32_t *seg_paths, const double *deg,
+                                                                     const T *chi_path, double *chi_seg) {
+    __shared__ double part[SEG_WAVES][FM_WAVE];
+    const int s = blockIdx.y;
+    const int w = threadIdx.x / FM_WAVE, lane = threadIdx.x % FM_WAVE;
+    const int k = blockIdx.x * FM_WAVE + lane;
+    double acc = 0.0;
+    if (k < nk) {
+        for (int t = seg_ptr[s] + w; t < seg_ptr[s + 1]; t += SEG_WAVES) {
+            const int p = seg_paths[t];
+            acc += deg[p] * (double)chi_path[(size_t)p * nk + k];
+        }
+    }
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && k < nk) {
+        double tot = 0.0;
+        for (int i = 0; i < SEG_WAVES; ++i) tot += part[i][lane];
+        chi_seg[(size_t)s * nk + k] = tot;
+    }
+}
 
 }  // namespace fm

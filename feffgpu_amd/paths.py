@@ -47,13 +47,12 @@ class Buckets:
     offset: np.ndarray         # (nb + 1,)
     kind: np.ndarray           # (nb,)
     edge: np.ndarray           # (nb,)
-    nlegmax: np.ndarray        # (nb,)
+    nleg: np.ndarray           # (nb,) legs of every path of the bucket
 
 
 def bucket_paths(nleg: np.ndarray, edge: np.ndarray, ilinit: list) -> Buckets:
-    """Group the paths by kernel variant -- (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (K/L1 edge, 2 legs) -- and by
-    edge; inside a bucket the paths are ordered by decreasing number of legs, so a workgroup's paths share their
-    number of legs and the tail of a launch is short."""
+    """Group the paths by kernel variant -- (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (K/L1 edge, 2 legs) --, by
+    edge and by number of legs: the kernels run the legs of a launch with wave-uniform loops."""
     nleg = np.asarray(nleg)
     edge = np.asarray(edge)
     il = np.asarray(ilinit)[edge]
@@ -62,14 +61,14 @@ def bucket_paths(nleg: np.ndarray, edge: np.ndarray, ilinit: list) -> Buckets:
     if (kind < 0).any():
         raise NotImplementedError("two-leg paths of L and M edges are not implemented on the GPU")
     order = np.lexsort((np.arange(len(nleg)), -nleg, edge, kind)).astype(np.int32)
-    key = (kind * 16 + edge)[order]
+    key = ((kind * 16 + edge) * 16 + (15 - nleg))[order]
     off, kinds, edges, nlm = [0], [], [], []
     for kv in np.unique(key):
         sel = key == kv
         off.append(off[-1] + int(sel.sum()))
-        kinds.append(int(kv) // 16)
-        edges.append(int(kv) % 16)
-        nlm.append(int(nleg[order][sel].max()))
+        kinds.append(int(kv) // 256)
+        edges.append(int(kv) // 16 % 16)
+        nlm.append(15 - int(kv) % 16)
     return Buckets(order, np.array(off, np.int32), np.array(kinds, np.int32), np.array(edges, np.int32),
                    np.array(nlm, np.int32))
 
@@ -198,7 +197,7 @@ class PathPlan:
         self._h_off = (N._i32 * len(self.buckets.offset))(*self.buckets.offset.tolist())
         self._h_kind = (N._i32 * max(1, len(self.buckets.kind)))(*self.buckets.kind.tolist())
         self._h_edge = (N._i32 * max(1, len(self.buckets.edge)))(*self.buckets.edge.tolist())
-        self._h_nlm = (N._i32 * max(1, len(self.buckets.nlegmax)))(*self.buckets.nlegmax.tolist())
+        self._h_nlm = (N._i32 * max(1, len(self.buckets.nleg)))(*self.buckets.nleg.tolist())
 
     def _paths_struct(self, pos64):
         s = N.FmPaths()
@@ -264,6 +263,33 @@ class PathPlan:
         """Differentiable evaluation: (chi_seg (nseg, NK) float64, chi_path (P, NK)) with gradients to pos."""
         return _ChiPathsFn.apply(torch.as_tensor(pos), self)
 
+    def forward_plain(self, pos, workspace_bytes: int = 16 << 30):
+        """The same list through the plain FP64 pipeline (fm_chi_paths_plain: the reference's op sequence, one kernel
+        per stage, every intermediate in HBM; chunks of as many paths as `workspace_bytes` holds). Returns (chi_seg,
+        chi_path, reff), float64, the plan's own buffers for this entry."""
+        pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
+        if self.max_atom >= pos64.shape[0]:
+            raise ValueError("pos has too few rows for the path list")
+        L = N.lib()
+        if getattr(self, "_plain", None) is None:
+            per = max((L.fm_plain_path_bytes(self.edges[int(e)].struct, int(n))
+                       for e, n in zip(self.buckets.edge, self.buckets.nleg)), default=1)
+            size = int(min(max(per, workspace_bytes), per * max(self.npath, 1)))
+            self._plain = (torch.empty(size, dtype=torch.uint8, device=self.device),
+                           torch.empty(self.npath, self.nk, dtype=torch.float64, device=self.device),
+                           torch.empty(self.npath, dtype=torch.float64, device=self.device))
+        ws, chi_path, reff = self._plain
+        chi_seg = torch.empty(self.nseg, self.nk, dtype=torch.float64, device=self.device)
+        if not self.npath:
+            return chi_seg.zero_(), chi_path, reff
+        ps = self._paths_struct(pos64)
+        N.check(L.fm_chi_paths_plain(self.edge_array, len(self.edges), ps, self.grid, self.iorder, self.order_t.data_ptr(),
+                                     len(self.buckets.kind), self._h_off, self._h_kind, self._h_edge, self._h_nlm,
+                                     self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(), chi_seg.data_ptr(),
+                                     chi_path.data_ptr(), reff.data_ptr(), ws.data_ptr(), ws.numel(),
+                                     N.stream_of(self.device)), "fm_chi_paths_plain")
+        return chi_seg, chi_path, reff
+
     def forward_bucket(self, pos, b):
         """Only bucket b of the plan (no segment sum): one kernel launch, for timing the kernel variants."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
@@ -272,7 +298,7 @@ class PathPlan:
         off = (N._i32 * 2)(int(self.buckets.offset[b]), int(self.buckets.offset[b + 1]))
         kind = (N._i32 * 1)(int(self.buckets.kind[b]))
         edge = (N._i32 * 1)(int(self.buckets.edge[b]))
-        nlm = (N._i32 * 1)(int(self.buckets.nlegmax[b]))
+        nlm = (N._i32 * 1)(int(self.buckets.nleg[b]))
         ps = self._paths_struct(pos64)
         N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
                                      self.order_t.data_ptr(), 1, off, kind, edge, nlm, 0, None, None, None,

@@ -152,38 +152,72 @@ class EdgeTables:
     chw: np.ndarray = None      # (nchan, E) complex
     rotc: np.ndarray = None
     rotn: np.ndarray = None
-    gpaths: int = 4
-    gstart: list = field(default_factory=list)
+    bands: list = field(default_factory=list)   # band plans for 1, 2, 4, 8 paths per workgroup (see energy_bands)
 
 
-def energy_bands(profile: np.ndarray, nbands: int = 4):
-    """Paths per workgroup and the split of the energies into `nbands` contiguous bands of at most 64 / paths
-    energies each, minimising sum over bands of (band size) x (largest partial-wave count in the band): the lanes of
-    a wavefront then loop over similar numbers of partial waves."""
+MAX_BANDS = 16
+WAVES = 4
+BAND_OVERHEAD = 6.0   # per-lane work of an energy outside the partial-wave rows (termination, chain, cfac), in rows
+
+
+def _splits(cost, cap: int, nbmax: int):
+    """Contiguous splits of the energies into nb = 1..nbmax bands of at most cap energies, each minimising the sum
+    over bands of the band's largest cost (a wavefront runs a band for as long as its most expensive lane).
+    Returns {nb: starts} for the nb that admit a split."""
+    E = len(cost)
+    # M[i, j] = max(cost[i:j]) for 0 < j - i <= cap, inf otherwise
+    M = np.full((E + 1, E + 1), np.inf)
+    for i in range(E):
+        hi = min(E, i + cap)
+        M[i, i + 1:hi + 1] = np.maximum.accumulate(cost[i:hi])
+    best = np.full(E + 1, np.inf)
+    best[0] = 0.0
+    cuts, out = [], {}
+    for nb in range(1, nbmax + 1):
+        tot = best[:, None] + M                       # (i, j): bands so far ending at i, then [i, j)
+        cut = np.argmin(tot, axis=0)                  # first minimum: deterministic
+        best = tot[cut, np.arange(E + 1)]
+        cuts.append(cut)
+        if np.isfinite(best[E]):
+            starts, j = [E], E
+            for c in reversed(cuts):
+                j = int(c[j])
+                starts.append(j)
+            out[nb] = starts[::-1]
+    return out
+
+
+def _assign(costs, nwave: int = WAVES):
+    """Longest-processing-time-first assignment of the bands to the wavefronts of a workgroup: (wave of each band,
+    the largest wave total). Ties go to the lowest wave index (deterministic)."""
+    load = [0.0] * nwave
+    wave = [0] * len(costs)
+    for b in sorted(range(len(costs)), key=lambda b: (-costs[b], b)):
+        w = min(range(nwave), key=lambda w: (load[w], w))
+        wave[b] = w
+        load[w] += costs[b]
+    return wave, max(load)
+
+
+def energy_bands(profile: np.ndarray):
+    """Band plans of a workgroup of WAVES wavefronts holding gp = 1, 2, 4 or 8 paths: the energies are split into at
+    most MAX_BANDS contiguous bands of at most 64 / gp energies (lane (path j, energy i) = j * (64 / gp) + i), and
+    each band is run by one wavefront, chosen so that the wavefronts' totals are balanced. profile[e] is the number
+    of partial-wave rows at energy e. Returns [(gp, starts, waves)] for gp = 1, 2, 4, 8; starts = [] where the
+    energies do not fit."""
     E = len(profile)
-    gp = 4 if E <= 64 else (2 if E <= 128 else 1)
-    cap = 64 // gp
-    if E > cap * nbands:
-        raise ValueError(f"{E} energies do not fit {nbands} bands of {cap}")
-    INF = float("inf")
-    # cost[i][j]: band [i, j)
-    best = [[INF] * (E + 1) for _ in range(nbands + 1)]
-    cut = [[0] * (E + 1) for _ in range(nbands + 1)]
-    best[0][0] = 0.0
-    for b in range(1, nbands + 1):
-        for j in range(E + 1):
-            for i in range(max(0, j - cap), j + 1):
-                if best[b - 1][i] == INF:
-                    continue
-                c = best[b - 1][i] + (float(profile[i:j].max()) * (j - i) if j > i else 0.0)
-                if c < best[b][j]:
-                    best[b][j], cut[b][j] = c, i
-    starts = [E]
-    j = E
-    for b in range(nbands, 0, -1):
-        j = cut[b][j]
-        starts.append(j)
-    return gp, starts[::-1]
+    cost = np.asarray(profile, float) + BAND_OVERHEAD
+    plans = []
+    for gp in (1, 2, 4, 8):
+        cap = 64 // gp
+        best = None
+        for nb, st in sorted(_splits(cost, cap, MAX_BANDS).items()):
+            bc = [float(cost[st[b]:st[b + 1]].max()) for b in range(nb)]
+            wave, span = _assign(bc)
+            if best is None or span < best[0] - 1e-9:
+                best = (span, st, wave)
+        plans.append((gp, best[1], best[2]) if best else (gp, [], []))
+    return plans
 
 
 def edge_tables(P: PhaseData, E: int | None = None) -> EdgeTables:
@@ -215,11 +249,10 @@ def edge_tables(P: PhaseData, E: int | None = None) -> EdgeTables:
             if lind[k] == l:
                 chw[c] += bdiag[k] * P.rkk[:E, k - 1, 0] ** 2
     rotc, rotn = rotation_coefficients(L1)
-    gpaths, gstart = energy_bands(lm.max(1) + 1)
     return EdgeTables(E=E, L1=L1, nph1=nph1, ilinit=linit + 1, kinit=kinit, lmax=np.ascontiguousarray(lm.T),
                       lmax1=(lm.max(0) + 1).astype(np.int32), ck=ck, ckmxk=ck - xk, kf=xk / BOHR,
                       kf_full=P.xk[:P.ne1] / BOHR, tl=tl, chan_il=[l + 1 for l in ls], chw=chw, rotc=rotc, rotn=rotn,
-                      gpaths=gpaths, gstart=gstart)
+                      bands=energy_bands(lm.max(1) + 1))
 
 
 def energies_needed(P: PhaseData, q) -> int:

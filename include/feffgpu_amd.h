@@ -28,6 +28,16 @@ extern "C" {
 
 enum { FM_OK = 0, FM_EINVAL = -1, FM_EUNSUPPORTED = -2, FM_EHIP = -3 };
 
+/* A split of the phase-grid energies into bands for a workgroup of 4 wavefronts holding gpaths paths: a band is
+ * run by one wavefront, lane (path j, energy i) = j * (64 / gpaths) + i, so the lanes of a wavefront loop over
+ * similar numbers of partial waves; the bands are spread over the wavefronts so that their totals are balanced. */
+typedef struct fm_bands {
+    int32_t nband;               /* 0 (no plan for this gpaths) or 1..16 */
+    int32_t start[17];           /* band b = energies [start[b], start[b+1]); start[0] = 0, start[nband] = E, each band
+                                    1..64 / gpaths energies */
+    int32_t wave[16];            /* the wavefront (0..3) that runs band b; a wavefront runs its bands in increasing b */
+} fm_bands;
+
 /* Device tables of one absorber edge, built once from a phase.pad (GPUPathStage.__init__, src/feffgpu/stage.py:97-125,
  * and the lambda / rotation bookkeeping of src/feffgpu/stage.py:41-93, 128-144). */
 typedef struct fm_edge {
@@ -48,9 +58,8 @@ typedef struct fm_edge {
     const double *chw64;
     const double *rotc;          /* [L1 + 1][49][3] d-recursion coefficients P, Q, R per l and (m1 + 3) * 7 + m2 + 3 */
     const double *rotn;          /* [L1 + 1][16] normalisation xnlm(il, |m1|) (-1)^|m1| / xnlm(il, |m2|), index |m1| * 4 + |m2| */
-    int32_t gpaths;              /* host: paths per workgroup (4 if E <= 64, 2 if E <= 128, else 1) */
-    int32_t gstart[5];           /* host: the 4 energy bands of a workgroup's waves, [gstart[w], gstart[w+1]), gstart[4] = E,
-                                    each at most 64 / gpaths energies (chosen so lanes of a band have similar lmax) */
+    fm_bands bands[4];           /* host: band plans for gpaths = 1, 2, 4, 8 (a launch takes the largest gpaths whose
+                                    paths fit the workgroup's LDS) */
 } fm_edge;
 
 /* A path list in the atom-index form of feffgpu's forward model (src/feffgpu/exafs/forward.py:143-163, 664-685). */
@@ -96,9 +105,10 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
                      const double *coords, const int64_t *ipots, const void *gcchi, const double *greff, double *gcoords,
                      void *workspace, size_t workspace_bytes, void *stream);
 
-/* Bytes of the workspace the adjoints need for `wave_slots` concurrently resident paths (each keeps its prefix
- * products there; the launch uses as many slots as the workspace holds, at least 4). No device call. */
-size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t wave_slots);
+/* Bytes of the workspace the adjoints need for `band_slots` energy bands of paths of at most nlegmax legs in flight
+ * at once (each lane keeps its scattering matrices, prefix products and gradient terms there; a workgroup holds one
+ * slot per band of its plan, at most 16, and a launch uses as many workgroups as the workspace holds). */
+size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t band_slots);
 
 /* GPUPathStage.chi_fine with the linear ("feff") convention (src/feffgpu/stage.py:365-388): chi [npath][nk] =
  * A(q) sin(Phi(q) + 2 q reff). cchi as written by fm_cchi; jq/fq of grid are read for edge 0. */
@@ -117,15 +127,15 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
  * order [npath] (device) is the processing order, a permutation of 0..npath-1 grouped into nbucket buckets:
  * bucket b holds order[bucket_offset[b] .. bucket_offset[b+1]) (host array), all of kind bucket_kind[b] (host):
  * FM_BUCKET_K_MS (K/L1 edge, 3..9 legs), FM_BUCKET_LM_MS (L2/L3/M edges, 3..9 legs), FM_BUCKET_K_SS (K/L1
- * edge, 2 legs); every path of bucket b has edge bucket_edge[b] (host, may be NULL: edge 0);
- * bucket_nlegmax[b] (host, may be NULL) is the largest nleg in the bucket (sizes LDS).
+ * edge, 2 legs); every path of bucket b has edge bucket_edge[b] (host, may be NULL: edge 0) and bucket_nleg[b]
+ * legs (host, 2..nmax; a path whose nleg differs gets NaN for its chi_path row and reff).
  * seg_ptr [nseg + 1] and seg_paths [npath] (device) list the paths of each segment (CSR). chi_path (float for
  * precision 32, double for 64) is required: it is also the workspace of the deterministic segment sum.
  * chi_seg (double) and reff (double, A) may be NULL. */
 enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_K_SS = 2 };
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
                  int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,
-                 const int32_t *bucket_kind, const int32_t *bucket_edge, const int32_t *bucket_nlegmax, int32_t nseg,
+                 const int32_t *bucket_kind, const int32_t *bucket_edge, const int32_t *bucket_nleg, int32_t nseg,
                  const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff,
                  void *stream);
 
@@ -133,14 +143,32 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
  * dL/dchi_seg (double) with segment [npath] the row of each path, gpath [npath][nk] = dL/dchi_path (element type by
  * precision) or NULL. Writes gcoord [npath][nmax][3] (double; dL/dcoords of each path's scatterers in slots
  * 1..nleg-1 and minus their sum, the absorber's share, in slot 0) and grad_pos [natom][3] = dL/dpos, the sum over
- * atom_ptr [natom + 1] / atom_refs (CSR of path * nmax + slot references, device), in list order. The forward is
- * recomputed inside (nothing from the forward call is kept). */
+ * atom_ptr [natom + 1] / atom_refs (CSR of path * nmax + slot references, device), in list order (a path whose
+ * nleg differs from its bucket's gets NaN). The forward is recomputed inside (nothing from the forward call is kept). */
 int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                           int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
-                          const int32_t *bucket_nlegmax, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
+                          const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
                           void *workspace, size_t workspace_bytes, void *stream);
+
+/* The plain FP64 path stage: the reference's op sequence (src/feffgpu/stage.py:147-388) with one kernel per stage
+ * and every intermediate (rotation matrices, spherical-wave factors, gam / gamtl, scattering matrices, termination,
+ * cchi) in HBM -- the comparator of the fused kernels on the same GPU, and an independent evaluation for tests.
+ * Same path list, buckets, grid and outputs as fm_chi_paths at precision 64 (chi_path, reff, chi_seg double); each
+ * bucket is processed in chunks of as many paths as `workspace` holds, fm_plain_path_bytes(edge, nleg) per path. */
+size_t fm_plain_path_bytes(const fm_edge *edge, int32_t nleg);
+int fm_chi_paths_plain(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t iorder,
+                       const int32_t *order, int32_t nbucket, const int32_t *bucket_offset, const int32_t *bucket_kind,
+                       const int32_t *bucket_edge, const int32_t *bucket_nleg, int32_t nseg, const int32_t *seg_ptr,
+                       const int32_t *seg_paths, double *chi_seg, double *chi_path, double *reff, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/* Launch policy (process-wide): the LDS in KiB (16..160) that a workgroup of kernel 0 (forward calls) or 1 (adjoint
+ * calls) may take when the number of paths per workgroup is chosen; defaults 80 (two forward workgroups per CU) and
+ * 160 (one adjoint workgroup per CU). Set by feffgpu_amd.tune from timings. Returns the previous value, or -1 for
+ * bad arguments. No counterpart in the reference (its stage has no launch configuration). */
+int32_t fm_set_lds_budget(int32_t kernel, int32_t kib);
 
 /* Message of the last failed call on this thread. */
 const char *fm_last_error(void);
