@@ -97,6 +97,29 @@ def cpu_baseline(pos, pl, q, cell, seconds):
                        f"(oracle/feffref.c, OpenMP), {t:.1f} s")
 
 
+def dominant_bucket(plan, pl):
+    """The bucket with the most algorithmic work and the per-bucket flops (roofline.path_cmacs x 8)."""
+    from feffgpu_amd.roofline import FLOPS_PER_CMAC, path_cmacs
+    cm = path_cmacs(plan.tables[0], plan.nleg_h, pl.ipots)
+    work = []
+    for b in range(len(plan.buckets.kind)):
+        sel = plan.buckets.order[plan.buckets.offset[b]:plan.buckets.offset[b + 1]]
+        work.append(float(cm[sel].sum() * FLOPS_PER_CMAC))
+    return int(np.argmax(work)), work
+
+
+def copy_only(torch, steps):
+    """A device-to-device copy of 1 GiB, `steps` times: the known traffic against which the FETCH_SIZE and
+    WRITE_SIZE counters are calibrated."""
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] copy of {4 * n} bytes x {steps}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +128,11 @@ def main():
     ap.add_argument("--workload", default="md")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-only", choices=["fwd", "bwd", "copy"], default=None,
+                    help="run only the dominant bucket's forward or adjoint kernel (or a plain device copy of known "
+                         "size, for calibrating counters) --steps times, for rocprofv3 --pmc passes")
+    ap.add_argument("--lds-kib", type=int, nargs=2, default=None, metavar=("FWD", "BWD"),
+                    help="launch policy (LDS budget per workgroup) instead of tuning")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -113,8 +141,10 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.kernel_only == "copy":
+        return copy_only(torch, args.steps)
     from feffgpu_amd.paths import PathPlan
-    from feffgpu_amd.roofline import FLOPS_PER_CMAC, MI355X_HBM_TBPS, MI355X_PEAK_TFLOPS, path_cmacs
+    from feffgpu_amd.roofline import MI355X_PEAK_TFLOPS
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a ROCm GPU")
@@ -130,9 +160,26 @@ def main():
     sync = torch.cuda.synchronize
     units = len(pl) * len(q)
     # launch policy (paths per workgroup) measured on this workload before anything is timed
-    from feffgpu_amd.tune import tune
-    policy = tune(plan32, pos_t)
+    from feffgpu_amd.tune import set_budget, tune
+    if args.lds_kib:
+        set_budget(0, args.lds_kib[0])
+        set_budget(1, args.lds_kib[1])
+        policy = dict(chosen=dict(forward=args.lds_kib[0], backward=args.lds_kib[1]), forward={}, backward={})
+    else:
+        policy = tune(plan32, pos_t)
     print(f"[bench] launch policy {policy}", file=sys.stderr, flush=True)
+    bdom, b_work = dominant_bucket(plan32, pl)
+    gseg_one = torch.ones(nseg, len(q), dtype=torch.float64, device=dev)
+    if args.kernel_only:
+        run = (lambda: plan32.forward_bucket(pos_t, bdom)) if args.kernel_only == "fwd" else \
+              (lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
+        for _ in range(args.steps):
+            run()
+        sync()
+        print(f"[bench] {args.kernel_only} of bucket {bdom} ({int(plan32.buckets.nleg[bdom])} legs, "
+              f"{int(plan32.buckets.offset[bdom + 1] - plan32.buckets.offset[bdom])} paths) x {args.steps}",
+              file=sys.stderr, flush=True)
+        return
 
     # parity of this workload: FP32 fused against FP64 fused (equal to the FP64 oracle to ~1e-14)
     s32 = plan32.forward(pos_t)[0].cpu().numpy()
@@ -168,32 +215,37 @@ def main():
     print(f"[bench] fp64 plain forward {ms_plain64:.3f} ms/step (vs fused fp64 max rel {plain_rel:.1e})",
           file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel: the bucket with the most work, timed alone with HIP events on its stream
-    cm = path_cmacs(plan32.tables[0], plan32.nleg_h, pl.ipots)
-    b_work = []
-    for b in range(len(plan32.buckets.kind)):
-        sel = plan32.buckets.order[plan32.buckets.offset[b]:plan32.buckets.offset[b + 1]]
-        b_work.append(cm[sel].sum() * FLOPS_PER_CMAC)
-    bdom = int(np.argmax(b_work))
+    # roofline of the dominant kernel of the step, the adjoint of the bucket with the most work (algorithmic flops
+    # = 2 x the forward's, recomputation not counted), and of that bucket's forward; HIP events on the launch stream
     stream = torch.cuda.current_stream(dev)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     nrep = max(3, args.steps)
-    for _ in range(2):
-        plan32.forward_bucket(pos_t, bdom)
-    ev[0].record(stream)
-    for _ in range(nrep):
-        plan32.forward_bucket(pos_t, bdom)
-    ev[1].record(stream)
-    sync()
-    t_dom = ev[0].elapsed_time(ev[1]) / nrep * 1e-3
-    achieved = b_work[bdom] / t_dom / 1e12
-    kind_names = {0: "K multiple scattering (3-4 legs)", 1: "L/M multiple scattering", 2: "K single scattering"}
-    roof = dict(bound="mfma", achieved=achieved, peak=MI355X_PEAK_TFLOPS["fp32"], unit="TFLOP/s",
-                frac=achieved / MI355X_PEAK_TFLOPS["fp32"], traffic=None,
-                kernel=f"k_forward<float,{int(plan32.buckets.kind[bdom])}> ({kind_names.get(int(plan32.buckets.kind[bdom]))})",
-                launch_ms=t_dom * 1e3, flops_per_launch=b_work[bdom],
-                note="FP32 vector pipes (no MFMA); 157.3 is MI355X's FP32 peak with packed math, which this build "
-                     "disables (non-packed VALU ceiling 78.6)")
+
+    def launch_ms(fn):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        for _ in range(2):
+            fn()
+        ev[0].record(stream)
+        for _ in range(nrep):
+            fn()
+        ev[1].record(stream)
+        sync()
+        return ev[0].elapsed_time(ev[1]) / nrep
+
+    kind_names = {0: "K multiple scattering", 1: "L/M multiple scattering", 2: "K single scattering"}
+    what = f"{kind_names.get(int(plan32.buckets.kind[bdom]))}, {int(plan32.buckets.nleg[bdom])} legs"
+    t_bwd = launch_ms(lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
+    t_fwd = launch_ms(lambda: plan32.forward_bucket(pos_t, bdom))
+    peak = MI355X_PEAK_TFLOPS["fp32"]
+    note = ("FP32 vector pipes (no MFMA); 157.3 is MI355X's FP32 peak with packed math, which this build disables "
+            "(non-packed VALU ceiling 78.6)")
+    ach_b = 2 * b_work[bdom] / (t_bwd * 1e-3) / 1e12
+    ach_f = b_work[bdom] / (t_fwd * 1e-3) / 1e12
+    roof = dict(bound="mfma", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak, traffic=None,
+                kernel=f"k_backward<float,{int(plan32.buckets.kind[bdom])},*> ({what})", launch_ms=t_bwd,
+                flops_per_launch=2 * b_work[bdom], note=note)
+    roof_fwd = dict(achieved=ach_f, peak=peak, unit="TFLOP/s", frac=ach_f / peak,
+                    kernel=f"k_forward<float,{int(plan32.buckets.kind[bdom])}> ({what})", launch_ms=t_fwd,
+                    flops_per_launch=b_work[bdom])
 
     out = dict(metric=METRIC, value=units / (ms_fb32 * 1e-3), unit="paths·k-pts/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=ms_fb32, higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -208,7 +260,7 @@ def main():
                parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
                                   backward_ms=policy["backward"]),
-               roofline=roof)
+               roofline=roof, roofline_forward=roof_fwd)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, pl, q, cell, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -307,6 +307,24 @@ class PathPlan:
         return self.chi_path
 
 
+    def backward_bucket(self, pos, gseg, b):
+        """The adjoint of bucket b only (gcoord of its paths; no per-atom sum): one kernel launch, for timing."""
+        pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
+        gs = torch.as_tensor(gseg).to(device=self.device, dtype=torch.float64).contiguous()
+        off = (N._i32 * 2)(int(self.buckets.offset[b]), int(self.buckets.offset[b + 1]))
+        kind = (N._i32 * 1)(int(self.buckets.kind[b]))
+        edge = (N._i32 * 1)(int(self.buckets.edge[b]))
+        nlm = (N._i32 * 1)(int(self.buckets.nleg[b]))
+        ps = self._paths_struct(pos64)
+        N.check(N.lib().fm_chi_paths_backward(self.edge_array, len(self.edges), ps, self.grid, self.precision,
+                                              self.iorder, self.order_t.data_ptr(), 1, off, kind, edge, nlm,
+                                              self.segment_t.data_ptr(), gs.data_ptr(), None, 0, None, None,
+                                              self.gcoord.data_ptr(), None, self.workspace.data_ptr(),
+                                              self.workspace.numel(), N.stream_of(self.device)),
+                "fm_chi_paths_backward")
+        return self.gcoord
+
+
 class _ChiPathsFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pos, plan):

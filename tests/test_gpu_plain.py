@@ -86,17 +86,17 @@ def test_plain_chunking_is_invisible():
 
 
 def test_bucket_leg_count_mismatch_gives_nan_rows():
-    """A bucket that claims 3 legs for 4-leg paths: those rows are NaN, every other row is unchanged."""
+    """A bucket that claims 4 legs for 3-leg paths (or 3 for 4): those rows are NaN, every other row is unchanged."""
     from feffgpu_amd import _native as N
     phase = GOLDEN / "validation" / "A018" / "phase.pad"
-    pos, box, pl = _box(14, [6], phase, rmax=4.4)
+    pos, box, pl = _box(14, [6], phase, rmax=4.8)
     plan = _plan(phase, pl, 32, box, 1)
     p = torch.tensor(pos, device="cuda")
     good = plan.forward(p)[1].clone()
-    b4 = [b for b in range(len(plan.buckets.nleg)) if plan.buckets.nleg[b] == 4]
+    b4 = [b for b in range(len(plan.buckets.nleg)) if plan.buckets.nleg[b] in (3, 4)]
     assert b4
     nlm = list(plan.buckets.nleg)
-    nlm[b4[0]] = 3
+    nlm[b4[0]] = 7 - nlm[b4[0]]
     h_nlm = (N._i32 * len(nlm))(*nlm)
     ps = plan._paths_struct(p)
     N.check(N.lib().fm_chi_paths(plan.edge_array, 1, ps, plan.grid, 32, 2, plan.order_t.data_ptr(),
