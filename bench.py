@@ -154,7 +154,8 @@ def main():
     print(f"[bench] workload {args.workload}: {len(pl)} paths, {len(q)} k-points, setup {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
     common = dict(segment=pl.segment, nseg=nseg, cell=cell, device=dev)
-    plan32 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
+    plan32 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32,
+                      save_intermediates=True, **common)
     plan64 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, **common)
     pos_t = torch.tensor(pos, device=dev, dtype=torch.float64)
     sync = torch.cuda.synchronize
@@ -167,6 +168,8 @@ def main():
         policy = dict(chosen=dict(forward=args.lds_kib[0], backward=args.lds_kib[1]), forward={}, backward={})
     else:
         policy = tune(plan32, pos_t)
+        policy64 = tune(plan64, pos_t, budgets=(80, 160))
+        print(f"[bench] FP64 launch policy {policy64['chosen']}", file=sys.stderr, flush=True)
     print(f"[bench] launch policy {policy}", file=sys.stderr, flush=True)
     bdom, b_work = dominant_bucket(plan32, pl)
     gseg_one = torch.ones(nseg, len(q), dtype=torch.float64, device=dev)

@@ -33,7 +33,6 @@ size_t fm_lds_budget(int kernel);
 namespace {
 
 __host__ __device__ constexpr int pre_elems(int set) { return lamdef(set).L * lamdef(set).L0; }   // one prefix product
-constexpr int FM_UPARK = 36 + FM_NREP;                              // parked: U (L0 x L <= 6 x 6), then K' (eta weights)
 __host__ __device__ inline int nacc(int nlegmax) { return 4 * nlegmax + 2; }   // gradient slots per lane
 
 // Gradient slots of a lane: kind X (leg length through the gam leg), Y (leg length through the gamtl leg), B (beta),
@@ -57,8 +56,10 @@ struct BArgs {
     int nslot, ltab, nk;
     int nlegmax;                // the leg count of every path of this launch
     size_t path_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_red;
-    unsigned char *work;        // per wave slot: prefix products, parked U, gradient slots
-    size_t work_per_wave, work_off_F, work_off_U, work_off_acc, work_bytes;
+    const void *saved;          // intermediates of the forward call ([npath][saved_rec] complex), or NULL
+    size_t saved_rec;
+    unsigned char *work;        // per band slot: prefix products and scattering matrices (unless saved), gradient slots
+    size_t work_per_wave, work_off_F, work_off_acc, work_bytes;
 };
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }
@@ -68,7 +69,7 @@ int hip_err(hipError_t e, const char *where) {
     return fm_set_err(FM_EHIP, buf);
 }
 
-template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed) {
+template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bool saved) {
     size_t o = align16(sizeof(PathLDS));
     const size_t tab = align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
     a.off_W = o; o += tab;
@@ -80,9 +81,8 @@ template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed) {
     a.off_red = o; o += align16(sizeof(double) * (nacc(a.nlegmax) + 6 * FM_MAX_LEGS));
     a.path_bytes = o;
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
-    a.work_off_F = sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE;
-    a.work_off_U = a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE;
-    a.work_off_acc = a.work_off_U + sizeof(cx<T>) * FM_UPARK * FM_WAVE;
+    a.work_off_F = saved ? 0 : sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE;
+    a.work_off_acc = saved ? 0 : a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE;
     a.work_per_wave = (a.work_off_acc + sizeof(T) * (size_t)nacc(a.nlegmax) * FM_WAVE + 255) & ~size_t(255);
 }
 
@@ -259,6 +259,55 @@ template <class T, int SET> struct PreStore {
 #pragma unroll
         for (int k = 0; k < FM_NREP; ++k) Ft[k] = fs.ld((s - 1) * FM_NREP + k);
     }
+    // Pre_s (rows L; L0 for s = n, whose slot holds Pre_n)
+    __device__ __forceinline__ void mat(int s, const PathLDS &, cx<T> (&P)[lamdef(SET).L][lamdef(SET).L0]) const {
+        constexpr int L = lamdef(SET).L, L0 = lamdef(SET).L0;
+#pragma unroll
+        for (int i = 0; i < L; ++i)
+#pragma unroll
+            for (int b = 0; b < L0; ++b) P[i][b] = get(s - 2, i * L0 + b);
+    }
+};
+
+// The same interface over the intermediates a forward call saved (save_f): F_s of this lane at
+// f[((s - 1) * FM_NREP + k) * stride]; Pre_s = F_{s-1} ... F_1 (with the row phases) is rebuilt from them.
+template <class T, int SET> struct PreSaved {
+    const cx<T> *f;
+    int stride;
+    __device__ __forceinline__ void get_f(int s, cx<T> (&Ft)[FM_NREP]) const {
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) Ft[k] = f[((size_t)(s - 1) * FM_NREP + k) * stride];
+    }
+    __device__ __forceinline__ void mat(int s, const PathLDS &g, cx<T> (&P)[lamdef(SET).L][lamdef(SET).L0]) const {
+        constexpr LamDef D = lamdef(SET);
+        constexpr PairDef PD = pairdef(SET);
+        constexpr int L = D.L, L0 = D.L0;
+        cx<T> Ft[FM_NREP], ph[6];
+        get_f(1, Ft);
+        row_phases<T, SET>(ph, g, 2);
+#pragma unroll
+        for (int i = 0; i < L; ++i)
+#pragma unroll
+            for (int b = 0; b < L0; ++b) P[i][b] = ph[i] * Ft[PD.idx[i][b]] * T(PD.sg[i][b]);
+#pragma unroll 1
+        for (int t = 2; t < s; ++t) {   // Pre_{t+1} = F_t Pre_t
+            get_f(t, Ft);
+            row_phases<T, SET>(ph, g, t + 1);
+#pragma unroll
+            for (int b = 0; b < L0; ++b) {
+                cx<T> col[L];
+#pragma unroll
+                for (int j = 0; j < L; ++j) col[j] = P[j][b];
+#pragma unroll
+                for (int i = 0; i < L; ++i) {
+                    cx<T> acc = mk<T>(T(0), T(0));
+#pragma unroll
+                    for (int j = 0; j < L; ++j) cfma(acc, Ft[PD.idx[i][j]] * T(PD.sg[i][j]), col[j]);
+                    P[i][b] = ph[i] * acc;
+                }
+            }
+        }
+    }
 };
 
 // Pass A: the amplitude at energy ee with the prefix products stored: slot s-2 holds Pre_s = F_{s-1} ... F_1
@@ -400,15 +449,16 @@ __device__ __forceinline__ cx<T> cfac_of(const fm_edge &ed, const PathLDS &g, in
     return cdiv(mk<T>((T)damp * cp, (T)damp * sp), ckn * (T)g.prod_r);
 }
 
-// One step of pass B for scatterer s with an NR x NC block: H = conj(pbar) (Pre_s U_s)^T, its contraction with
-// the tangents of F_s, eta_{s+1}, and U <- U F_s (unless last). With the register budget of 2 wavefronts per SIMD, U
-// is parked in the workspace while F_s is streamed and F_s is fetched afterwards; with the whole register file (1
-// per SIMD) both stay in registers and F_s is fetched before the stream, so its latency is hidden.
-template <class T, int SET, int NR, int NC, int WPE>
+// One step of pass B for scatterer s with an NR x NC block, with U = U_s = T F_{n-1} ... F_{s+1}:
+//   K[rep]  = sum over the orbit of H[l1][l2] ph[l1] sg, H[l1][l2] = conj(pbar) M[l2][l1], M = Pre_s U (NC x NR);
+//   Kp[rep] = the same sum weighted by m_{l1}, so that dL/deta_{s+1} = Im sum_rep Kp[rep] Ft[rep];
+//   U <- U diag(ph) F_s (unless last), with F_s as pass A stored it;
+// then F_s is streamed over the partial waves for the tangents of its two leg lengths and its angle, contracted
+// with K. Only K and U are live across the stream.
+template <class T, int SET, int NR, int NC, int WPE, class Src>
 __device__ __forceinline__ void chain_step(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab,
                                            int ee, int s, cx<T> pbc, cx<T> (&U)[lamdef(SET).L0][lamdef(SET).L],
-                                           PreStore<T, SET> pre, LaneArray<cx<T>> upark, bool has_pre, bool last,
-                                           GSlots<T> gs) {
+                                           const Src &pre, bool has_pre, bool last, GSlots<T> gs) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L0 = D.L0;
@@ -416,39 +466,59 @@ __device__ __forceinline__ void chain_step(const fm_edge &ed, const PathLDS &g, 
     const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
     cx<T> ph[6];
     row_phases<T, SET>(ph, g, s + 1);
-    // K[rep] = sum over the orbit of H[l1][l2] ph[l1] sg, H[l1][l2] = pbc * M[l2][l1], M = Pre_s U (NC x NR); Kp the
-    // same sum weighted by m_{l1}: dL/deta_{s+1} = Im sum_rep Kp[rep] Ft[rep]
-    cx<T> K[FM_NREP], Kp[FM_NREP];
+    cx<T> K[FM_NREP];
+    {
+        cx<T> Kp[FM_NREP];
+        cx<T> P[D.L][L0];
+        if (has_pre) pre.mat(s, g, P);
 #pragma unroll
-    for (int k = 0; k < FM_NREP; ++k) K[k] = Kp[k] = mk<T>(T(0), T(0));
+        for (int k = 0; k < FM_NREP; ++k) K[k] = Kp[k] = mk<T>(T(0), T(0));
 #pragma unroll
-    for (int l2 = 0; l2 < NC; ++l2) {
-        cx<T> prow[L0];
+        for (int l2 = 0; l2 < NC; ++l2) {
+            cx<T> prow[L0];
 #pragma unroll
-        for (int a = 0; a < L0; ++a)
-            prow[a] = has_pre ? pre.get(s - 2, l2 * L0 + a) : (a == l2 ? mk<T>(T(1), T(0)) : mk<T>(T(0), T(0)));
+            for (int a = 0; a < L0; ++a)
+                prow[a] = has_pre ? P[l2][a] : (a == l2 ? mk<T>(T(1), T(0)) : mk<T>(T(0), T(0)));
 #pragma unroll
-        for (int l1 = 0; l1 < NR; ++l1) {
-            cx<T> m = mk<T>(T(0), T(0));
+            for (int l1 = 0; l1 < NR; ++l1) {
+                cx<T> m = mk<T>(T(0), T(0));
 #pragma unroll
-            for (int a = 0; a < L0; ++a) cfma(m, prow[a], U[a][l1]);
-            const cx<T> h = pbc * m * ph[l1];
-            const T hs = T(PD.sg[l1][l2] * D.M[l1]);
-            if (PD.sg[l1][l2] > 0) K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] + h;
-            else K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] - h;
-            if (D.M[l1] != 0) rfma(Kp[PD.idx[l1][l2]], h, hs);
+                for (int a = 0; a < L0; ++a) cfma(m, prow[a], U[a][l1]);
+                const cx<T> h = pbc * m * ph[l1];
+                const T hs = T(PD.sg[l1][l2] * D.M[l1]);
+                if (PD.sg[l1][l2] > 0) K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] + h;
+                else K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] - h;
+                if (D.M[l1] != 0) rfma(Kp[PD.idx[l1][l2]], h, hs);
+            }
         }
-    }
-    cx<T> Ft[FM_NREP];
-    if constexpr (WPE == 1) {
+        cx<T> Ft[FM_NREP];
         pre.get_f(s, Ft);
-    } else {
+        // eta_{s+1}: sum over (l1, l2) of m_{l1} Im(H F), F = ph sg Ft
+        T geta = T(0);
 #pragma unroll
-        for (int a = 0; a < L0; ++a)
+        for (int k = 0; k < FM_NREP; ++k)
+            if (PD.ri[k] < NR && PD.rj[k] < NC) geta += Kp[k].r * Ft[k].i + Kp[k].i * Ft[k].r;
+        gs.put(GK_H, s + 1, geta);
+        if (!last) {
 #pragma unroll
-            for (int l1 = 0; l1 < NR; ++l1) upark.st(a * 6 + l1, U[a][l1]);
+            for (int a = 0; a < L0; ++a) {
+                cx<T> row[6];
 #pragma unroll
-        for (int k = 0; k < FM_NREP; ++k) upark.st(36 + k, Kp[k]);
+                for (int i = 0; i < 6; ++i) row[i] = (i < NR) ? U[a][i] * ph[i] : mk<T>(T(0), T(0));
+#pragma unroll
+                for (int j = 0; j < D.L; ++j) {
+                    cx<T> v = mk<T>(T(0), T(0));
+                    if (j < NC) {
+#pragma unroll
+                        for (int i = 0; i < NR; ++i) {
+                            const cx<T> f = Ft[PD.idx[i][j]];
+                            if (PD.sg[i][j] > 0) cfma(v, row[i], f); else cfma(v, row[i], mk<T>(-f.r, -f.i));
+                        }
+                    }
+                    U[a][j] = v;
+                }
+            }
+        }
     }
     const int ip = g.ipot[s];
     const int ilx = ed.lmax[(size_t)ip * E + ee] + 1;
@@ -459,47 +529,12 @@ __device__ __forceinline__ void chain_step(const fm_edge &ed, const PathLDS &g, 
     gs.put(GK_X, s + 1, gX);
     gs.put(GK_Y, s, gY);
     gs.put(GK_B, s, gB);
-    if constexpr (WPE != 1) {
-        pre.get_f(s, Ft);
-#pragma unroll
-        for (int a = 0; a < L0; ++a)
-#pragma unroll
-            for (int l1 = 0; l1 < NR; ++l1) U[a][l1] = upark.ld(a * 6 + l1);
-#pragma unroll
-        for (int k = 0; k < FM_NREP; ++k) Kp[k] = upark.ld(36 + k);
-    }
-    // eta_{s+1}: sum over (l1, l2) of m_{l1} Im(H F), F = ph sg Ft
-    T geta = T(0);
-#pragma unroll
-    for (int k = 0; k < FM_NREP; ++k)
-        if (PD.ri[k] < NR && PD.rj[k] < NC) geta += Kp[k].r * Ft[k].i + Kp[k].i * Ft[k].r;
-    gs.put(GK_H, s + 1, geta);
-    if (!last) {
-#pragma unroll
-        for (int a = 0; a < L0; ++a) {
-            cx<T> row[6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) row[i] = (i < NR) ? U[a][i] * ph[i] : mk<T>(T(0), T(0));
-#pragma unroll
-            for (int j = 0; j < D.L; ++j) {
-                cx<T> v = mk<T>(T(0), T(0));
-                if (j < NC) {
-#pragma unroll
-                    for (int i = 0; i < NR; ++i) {
-                        const cx<T> f = Ft[PD.idx[i][j]];
-                        if (PD.sg[i][j] > 0) cfma(v, row[i], f); else cfma(v, row[i], mk<T>(-f.r, -f.i));
-                    }
-                }
-                U[a][j] = v;
-            }
-        }
-    }
 }
 
 // Pass B at energy ee: cbar = dL/dcchi, cc = cchi.
-template <class T, int SET, int WPE>
+template <class T, int SET, int WPE, class Src>
 __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee, int n,
-                       cx<T> cbar, cx<T> cc, PreStore<T, SET> pre, LaneArray<cx<T>> upark, GSlots<T> gs) {
+                       cx<T> cbar, cx<T> cc, const Src &pre, GSlots<T> gs) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
@@ -515,10 +550,11 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
     const cx<T> cf = cfac_of<T>(ed, g, ee, n);
     const cx<T> pbc = conj(cbar * conj(cf));          // conj(dL/dptrac)
     // ---- termination: H_T[a][b] = pbc Pre_n[b][a]; tangents of leg 1 (X), leg n (Y), beta_n; eta_1
-    const int pn_slot = n > 2 ? n - 2 : 0;
     const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
     cx<T> U[L0][L];
     {
+        cx<T> Pn[L][L0];   // Pre_n (rows L0)
+        pre.mat(n, g, Pn);
         cx<T> ph[6];
         row_phases<T, SET>(ph, g, 1);
         cx<T> K[FM_NREP];
@@ -528,7 +564,7 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
         for (int a = 0; a < L0; ++a)
 #pragma unroll
             for (int b = 0; b < L0; ++b) {
-                const cx<T> h = pbc * pre.get(pn_slot, b * L0 + a) * ph[a];
+                const cx<T> h = pbc * Pn[b][a] * ph[a];
                 if (PD.sg[a][b] > 0) K[PD.idx[a][b]] = K[PD.idx[a][b]] + h;
                 else K[PD.idx[a][b]] = K[PD.idx[a][b]] - h;
             }
@@ -570,7 +606,7 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
                 const cx<T> t = (b < L0) ? ph[a] * Tt[PD.idx[a][b]] * T(PD.sg[a][b]) : mk<T>(T(0), T(0));
                 U[a][b] = t;
                 if (b < L0 && D.M[a] != 0) {
-                    const cx<T> hf = pbc * pre.get(pn_slot, b * L0 + a) * t;
+                    const cx<T> hf = pbc * Pn[b][a] * t;
                     geta += T(D.M[a]) * hf.i;
                 }
             }
@@ -578,13 +614,13 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
     }
     // ---- the chain, s = n-1 .. 1
     if (n == 2) {
-        chain_step<T, SET, L0, L0, WPE>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, upark, false, true, gs);
+        chain_step<T, SET, L0, L0, WPE, Src>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, false, true, gs);
     } else {
-        chain_step<T, SET, L0, L, WPE>(ed, g, W, Wd, ltab, ee, n - 1, pbc, U, pre, upark, true, false, gs);
+        chain_step<T, SET, L0, L, WPE, Src>(ed, g, W, Wd, ltab, ee, n - 1, pbc, U, pre, true, false, gs);
 #pragma unroll 1
         for (int s = n - 2; s >= 2; --s)
-            chain_step<T, SET, L, L, WPE>(ed, g, W, Wd, ltab, ee, s, pbc, U, pre, upark, true, false, gs);
-        chain_step<T, SET, L, L0, WPE>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, upark, false, true, gs);
+            chain_step<T, SET, L, L, WPE, Src>(ed, g, W, Wd, ltab, ee, s, pbc, U, pre, true, false, gs);
+        chain_step<T, SET, L, L0, WPE, Src>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, false, true, gs);
     }
 }
 
@@ -682,7 +718,7 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
     return wave_sum(rb);
 }
 
-template <class T, int SET, int WPE>
+template <class T, int SET, int WPE, bool SAVED>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_backward(BArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int pidx[8], pval[8];
@@ -726,17 +762,21 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             }
         }
         __syncthreads();
-        // ---- 2. pass A over this wavefront's bands
+        // ---- 2. pass A over this wavefront's bands (or the forward call's saved cchi)
         for (int b = 0; b < nband; ++b) {
             if (a.bp.wave[b] != wid) continue;
-            unsigned char *wb = wg_work + (size_t)b * wpw;
-            const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
-                                       {slice_rsrc(wb + a.work_off_F, a.work_off_U - a.work_off_F), loc}};
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
             const int e = e0 + (il < nb ? il : nb - 1);
-            const cx<T> pt = pass_A<T, SET>(ed, gj, Wj, ltab, e, n, pre);
-            const cx<T> c = pt * cfac_of<T>(ed, gj, e, n);
-            if (il < nb) ccj[e] = c;
+            if constexpr (SAVED) {
+                if (il < nb) ccj[e] = reinterpret_cast<const cx<T> *>(a.saved)[(size_t)pidx[jl] * a.saved_rec + e];
+            } else {
+                unsigned char *wb = wg_work + (size_t)b * wpw;
+                const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
+                                           {slice_rsrc(wb + a.work_off_F, a.work_off_acc - a.work_off_F), loc}};
+                const cx<T> pt = pass_A<T, SET>(ed, gj, Wj, ltab, e, n, pre);
+                const cx<T> c = pt * cfac_of<T>(ed, gj, e, n);
+                if (il < nb) ccj[e] = c;
+            }
         }
         __syncthreads();
         // ---- 3. dL/dcchi per energy (path lists: the fine grid's adjoint), one wavefront per path
@@ -764,9 +804,6 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
         for (int b = 0; b < nband; ++b) {
             if (a.bp.wave[b] != wid) continue;
             unsigned char *wb = wg_work + (size_t)b * wpw;
-            const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
-                                       {slice_rsrc(wb + a.work_off_F, a.work_off_U - a.work_off_F), loc}};
-            const LaneArray<cx<T>> upark{slice_rsrc(wb + a.work_off_U, a.work_off_acc - a.work_off_U), loc};
             const GSlots<T> gs{{slice_rsrc(wb + a.work_off_acc, a.work_per_wave - a.work_off_acc), lor}, n};
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
             const int e = e0 + (il < nb ? il : nb - 1);
@@ -780,7 +817,14 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                     cbar = cbj[e];
                 }
             }
-            pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, upark, gs);
+            if constexpr (SAVED) {
+                const PreSaved<T, SET> pre{reinterpret_cast<const cx<T> *>(a.saved) + (size_t)pidx[jl] * a.saved_rec + E + e, E};
+                pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, gs);
+            } else {
+                const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
+                                           {slice_rsrc(wb + a.work_off_F, a.work_off_acc - a.work_off_F), loc}};
+                pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, gs);
+            }
         }
         __syncthreads();
         // ---- 5. per path: the gradient slots summed over the path's lanes of every band (fixed order), the geometry
@@ -907,11 +951,22 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, i
 // Paths per workgroup: the largest plan within the LDS budget (by default one workgroup per CU, whose 4 wavefronts
 // then have the whole register file: 1 wavefront per SIMD); a plan small enough for two workgroups per CU runs the
 // variant sized for 2 wavefronts per SIMD (FP32).
+template <class T, int SET, int WPE, bool SAVED> hipError_t launch_k(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
+    if (shm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_backward<T, SET, WPE, SAVED>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_backward<T, SET, WPE, SAVED>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
+    return hipGetLastError();
+}
+
 template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
     a.ltab = ed.L1 + 1;
-    layout<T, SET>(a, ed);
-    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(1) - 256);
+    const bool saved = a.saved != nullptr;
+    layout<T, SET>(a, ed, saved);
+    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3) - 256);
     if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
     if (g < 0) return hipErrorInvalidValue;
     a.gp = 1 << g;
@@ -922,21 +977,11 @@ template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st)
     long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
     if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
     if (blocks == 0) return hipSuccess;
-    const void *fn = (const void *)k_backward<T, SET, 1>;
-    if constexpr (sizeof(T) == 4)
-        if (shm <= FM_LDS_SHARE - 256) fn = (const void *)k_backward<T, SET, 2>;
-    if (shm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        if (e != hipSuccess) return e;
-    }
     if constexpr (sizeof(T) == 4) {
-        if (fn == (const void *)k_backward<T, SET, 2>) {
-            hipLaunchKernelGGL((k_backward<T, SET, 2>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
-            return hipGetLastError();
-        }
+        if (shm <= FM_LDS_SHARE - 256)
+            return saved ? launch_k<T, SET, 2, true>(a, blocks, shm, st) : launch_k<T, SET, 2, false>(a, blocks, shm, st);
     }
-    hipLaunchKernelGGL((k_backward<T, SET, 1>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
-    return hipGetLastError();
+    return saved ? launch_k<T, SET, 1, true>(a, blocks, shm, st) : launch_k<T, SET, 1, false>(a, blocks, shm, st);
 }
 
 template <class T> hipError_t dispatch_b(BArgs &a, int set, hipStream_t st) {
@@ -964,11 +1009,16 @@ int set_of(const fm_edge &ed, int nleg) {
 
 extern "C" {
 
+size_t fm_saved_bytes(int32_t precision, int32_t nmax, int32_t emax, int32_t npath) {
+    if (nmax < 2 || nmax > FM_MAX_LEGS || emax < 2 || emax > FM_MAX_E || npath < 0) return 0;
+    return (precision == 32 ? 8 : 16) * (size_t)(1 + (nmax - 1) * FM_NREP) * emax * (size_t)npath;
+}
+
 size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t band_slots) {
     if (nlegmax < 2 || nlegmax > FM_MAX_LEGS || band_slots < 1) return 0;
     const size_t ec = precision == 32 ? 8 : 16, er = ec / 2;
     const size_t nslots = nlegmax > 2 ? nlegmax - 1 : 1;
-    const size_t cxe = nslots * 36 + (size_t)(nlegmax - 1) * FM_NREP + FM_UPARK;
+    const size_t cxe = nslots * 36 + (size_t)(nlegmax - 1) * FM_NREP;
     const size_t per = (ec * cxe * FM_WAVE + er * (size_t)nacc(nlegmax) * FM_WAVE + 255) & ~size_t(255);
     return per * (size_t)band_slots;
 }
@@ -1042,7 +1092,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
                           const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          void *workspace, size_t workspace_bytes, void *stream) {
+                          const void *saved, size_t saved_bytes, void *workspace, size_t workspace_bytes, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     int emax = 0;
     for (int i = 0; i < nedge; ++i) {
@@ -1059,6 +1109,9 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     if (grid->qstride < emax + 1) return set_err(FM_EINVAL, "grid.qstride < E + 1");
     if (!workspace || workspace_bytes < fm_backward_workspace_bytes(precision, paths->nmax, 16))
         return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nmax, 16)");
+    const size_t rec = (size_t)(1 + (paths->nmax - 1) * FM_NREP) * emax;
+    if (saved && saved_bytes < fm_saved_bytes(precision, paths->nmax, emax, paths->npath))
+        return set_err(FM_EINVAL, "saved smaller than fm_saved_bytes(precision, nmax, largest E, npath)");
     if (nbucket > 0 && (!bucket_offset || !bucket_kind || !bucket_nleg)) return set_err(FM_EINVAL, "NULL bucket arrays");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {
@@ -1089,6 +1142,8 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         if ((set == 2) != (a.nlegmax == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
         a.work = (unsigned char *)workspace;
         a.work_bytes = workspace_bytes;
+        a.saved = saved;
+        a.saved_rec = rec;
         hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
         if (e == hipErrorInvalidValue)
             return set_err(FM_EINVAL, "fm_chi_paths_backward: the bucket's paths do not fit one workgroup's LDS");

@@ -295,10 +295,21 @@ template <> struct TL<double> {
     static __device__ __forceinline__ const double *chw(const fm_edge &e) { return e.chw64; }
 };
 
+// Stores the FM_NREP representatives of F_s (s = 1..n-1) at fs[((s - 1) * FM_NREP + k) * stride] when fs is set:
+// the intermediates an adjoint can reuse instead of recomputing the amplitude.
+template <class T, bool SAVE>
+__device__ __forceinline__ void save_f(cx<T> *fs, int stride, int s, const cx<T> (&Ft)[FM_NREP]) {
+    if (!SAVE || !fs) return;
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k) fs[((size_t)(s - 1) * FM_NREP + k) * stride] = Ft[k];
+}
+
 // cchi at energy e of the path held in g (tables in W). All lanes of the wave run it (e may be out of range for
-// some; they compute garbage that the caller drops, which keeps the loops uniform).
-template <class T, int SET>
-__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n) {
+// some; they compute garbage that the caller drops, which keeps the loops uniform). With fs set, every F_s is also
+// stored (save_f).
+template <class T, int SET, bool SAVE = false>
+__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n, cx<T> *fs = nullptr,
+                           int fstride = 0) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
@@ -379,6 +390,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
         const T *tl, *Wl;
         scat(1, ilx, tl, Wl);
         fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
+        save_f<T, SAVE>(fs, fstride, 1, Ft);
         row_phases<T, SET>(ph, g, 2);
 #pragma unroll
         for (int a = 0; a < L0; ++a)
@@ -394,6 +406,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(n - 1, ilx, tl, Wl);
             fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
+            save_f<T, SAVE>(fs, fstride, n - 1, Ft);
             row_phases<T, SET>(ph, g, n);
             chain(std::integral_constant<int, L0>{});
         }
@@ -403,6 +416,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(s, ilx, tl, Wl);
             fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
+            save_f<T, SAVE>(fs, fstride, s, Ft);
             row_phases<T, SET>(ph, g, s + 1);
             chain(std::integral_constant<int, L>{});
         }
@@ -412,6 +426,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(1, ilx, tl, Wl);
             fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
+            save_f<T, SAVE>(fs, fstride, 1, Ft);
             row_phases<T, SET>(ph, g, 2);
 #pragma unroll
             for (int a = 0; a < L0; ++a)

@@ -4,7 +4,7 @@ The kernels put G paths in a workgroup (1, 2, 4 or 8; bands of energies per wave
 More paths per workgroup balance the wavefronts better but take more LDS, which lowers the workgroups a CU holds and,
 for the adjoint, decides between the register budgets of 1 and 2 wavefronts per SIMD. Which is faster depends on the
 edge, the legs and the GPU, so it is measured: ``tune(plan, pos)`` times the plan's forward and adjoint under each
-candidate LDS budget and keeps the fastest for each (fm_set_lds_budget; process-wide).
+candidate LDS budget and keeps the fastest for each (fm_set_lds_budget; process-wide, per kernel and precision).
 """
 from __future__ import annotations
 
@@ -27,8 +27,9 @@ def _time(fn, reps: int) -> float:
     return ev[0].elapsed_time(ev[1]) / reps
 
 
-def set_budget(kernel: int, kib: int) -> int:
-    prev = N.lib().fm_set_lds_budget(int(kernel), int(kib))
+def set_budget(kernel: int, kib: int, precision: int = 32) -> int:
+    """kernel 0 (forward) or 1 (adjoint) of the given precision."""
+    prev = N.lib().fm_set_lds_budget(int(kernel) + (2 if precision == 64 else 0), int(kib))
     if prev < 0:
         raise ValueError("kernel must be 0 (forward) or 1 (adjoint) and the budget 16..160 KiB")
     return prev
@@ -42,9 +43,9 @@ def tune(plan, pos, budgets=BUDGETS_KIB, reps: int = 3) -> dict:
     out = {"forward": {}, "backward": {}, "chosen": {}}
     for kernel, name, fn in ((0, "forward", lambda: plan.forward(pos)), (1, "backward", lambda: plan.backward(pos, gseg))):
         for kib in budgets:
-            set_budget(kernel, kib)
+            set_budget(kernel, kib, plan.precision)
             out[name][kib] = _time(fn, reps)
         best = min(out[name], key=out[name].get)
-        set_budget(kernel, best)
+        set_budget(kernel, best, plan.precision)
         out["chosen"][name] = best
     return out
