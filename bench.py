@@ -174,6 +174,8 @@ def main():
     bdom, b_work = dominant_bucket(plan32, pl)
     gseg_one = torch.ones(nseg, len(q), dtype=torch.float64, device=dev)
     if args.kernel_only:
+        if args.kernel_only == "bwd":
+            plan32.forward(pos_t)      # the intermediates the adjoint of the timed step reads
         run = (lambda: plan32.forward_bucket(pos_t, bdom)) if args.kernel_only == "fwd" else \
               (lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
         for _ in range(args.steps):

@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libfeffgpu_amd.so"
 
 FM_MAX_LEGS, FM_MAX_CHAN, FM_MAX_POT, FM_NREP = 9, 4, 16, 20
-FM_BUCKET_K_MS, FM_BUCKET_LM_MS, FM_BUCKET_K_SS = 0, 1, 2
+FM_BUCKET_K_MS, FM_BUCKET_LM_MS, FM_BUCKET_SS = 0, 1, 2
 FM_OK, FM_EINVAL, FM_EUNSUPPORTED, FM_EHIP = 0, -1, -2, -3
 
 _vp = ctypes.c_void_p

@@ -71,7 +71,7 @@ int hip_err(hipError_t e, const char *where) {
 
 template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bool saved) {
     size_t o = align16(sizeof(PathLDS));
-    const size_t tab = align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
+    const size_t tab = SET >= FM_SET_SS ? 0 : align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
     a.off_W = o; o += tab;
     a.off_Wd = o; o += tab;
     a.off_cc = o; o += align16(sizeof(cx<T>) * ed.E);
@@ -81,8 +81,9 @@ template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bo
     a.off_red = o; o += align16(sizeof(double) * (nacc(a.nlegmax) + 6 * FM_MAX_LEGS));
     a.path_bytes = o;
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
-    a.work_off_F = saved ? 0 : sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE;
-    a.work_off_acc = saved ? 0 : a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE;
+    const bool store = !saved && SET < FM_SET_SS;   // single scattering keeps nothing but the gradient slots
+    a.work_off_F = store ? sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE : 0;
+    a.work_off_acc = store ? a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE : 0;
     a.work_per_wave = (a.work_off_acc + sizeof(T) * (size_t)nacc(a.nlegmax) * FM_WAVE + 255) & ~size_t(255);
 }
 
@@ -435,20 +436,6 @@ __device__ __forceinline__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, con
     return ptrac;
 }
 
-// cfac = exp(i R (ck - xk)) / prod (ck r_j), as the forward computes it
-template <class T>
-__device__ __forceinline__ cx<T> cfac_of(const fm_edge &ed, const PathLDS &g, int ee, int n) {
-    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
-    double phs = g.R * ed.ckmxk[2 * ee];
-    phs -= FM_TWO_PI * rint(phs / FM_TWO_PI);
-    const double damp = exp(-g.R * ed.ckmxk[2 * ee + 1]);
-    T sp, cp;
-    if constexpr (sizeof(T) == 4) sincosf((float)phs, &sp, &cp); else sincos(phs, &sp, &cp);
-    cx<T> ckn = ck;
-    for (int j = 1; j < n; ++j) ckn = ckn * ck;
-    return cdiv(mk<T>((T)damp * cp, (T)damp * sp), ckn * (T)g.prod_r);
-}
-
 // One step of pass B for scatterer s with an NR x NC block, with U = U_s = T F_{n-1} ... F_{s+1}:
 //   K[rep]  = sum over the orbit of H[l1][l2] ph[l1] sg, H[l1][l2] = conj(pbar) M[l2][l1], M = Pre_s U (NC x NR);
 //   Kp[rep] = the same sum weighted by m_{l1}, so that dL/deta_{s+1} = Im sum_rep Kp[rep] Ft[rep];
@@ -547,7 +534,7 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
         gs.put_slot(4 * gs.nlegmax, -wdk.i);
         gs.put_slot(4 * gs.nlegmax + 1, w.r);
     }
-    const cx<T> cf = cfac_of<T>(ed, g, ee, n);
+    const cx<T> cf = cfac_at<T>(ed, g, ee, n);
     const cx<T> pbc = conj(cbar * conj(cf));          // conj(dL/dptrac)
     // ---- termination: H_T[a][b] = pbc Pre_n[b][a]; tangents of leg 1 (X), leg n (Y), beta_n; eta_1
     const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
@@ -622,6 +609,40 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
             chain_step<T, SET, L, L, WPE, Src>(ed, g, W, Wd, ltab, ee, s, pbc, U, pre, true, false, gs);
         chain_step<T, SET, L, L0, WPE, Src>(ed, g, W, Wd, ltab, ee, 1, pbc, U, pre, false, true, gs);
     }
+}
+
+// Pass B of a single-scattering path (fm_ss.h) at energy ee: the amplitude with dual numbers in the common leg
+// length r. Its derivative goes to the leg-1 slot (both legs join the same two atoms, so d r_1 = d r_2 for every
+// displacement); the angles are pi for every position (zero slots for beta); the eta slots as for any path.
+template <class T, int ILI>
+__device__ __forceinline__ void ss_pass_B(const fm_edge &ed, const PathLDS &g, int ee, cx<T> cbar, cx<T> cc, GSlots<T> gs) {
+    const int E = ed.E;
+    {
+        const cx<T> w = conj(cbar) * cc;
+        const cx<T> dk = mk<T>((T)ed.ckmxk[2 * ee], (T)ed.ckmxk[2 * ee + 1]);
+        const cx<T> wdk = w * dk;
+        gs.put_slot(4 * gs.nlegmax, -wdk.i);
+        gs.put_slot(4 * gs.nlegmax + 1, w.r);
+    }
+    const cx<T> pbc = conj(cbar * conj(cfac_at<T>(ed, g, ee, 2)));
+    const int lxa = max(ed.lmax[(size_t)g.ipot[0] * E + ee], ed.lmax[(size_t)g.ipot[1] * E + ee]) + 1;
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    const T r = (T)g.r[1];
+    const cx<T> z = minus_i_over(ck * r);
+    dcx<T> Tk[SS_MAXR], Fk[SS_MAXR];
+    ss_reps<T, ILI, dcx<T>>(ed, ee, g.ipot[1], dcx<T>{z, z * (T(-1) / r)}, lxa, Tk, Fk);
+    const dcx<T> pt = ss_trace<T, ILI, dcx<T>>(g.eta, Tk, Fk);
+    const cx<T> gr = pbc * pt.d;
+    gs.put(GK_X, 1, gr.r);
+    gs.put(GK_X, 2, T(0));
+    gs.put(GK_Y, 1, T(0));
+    gs.put(GK_Y, 2, T(0));
+    gs.put(GK_B, 1, T(0));
+    gs.put(GK_B, 2, T(0));
+    T g1, g2;
+    ss_trace_eta<T, ILI, dcx<T>>(g.eta, Tk, Fk, pbc, g1, g2);
+    gs.put(GK_H, 1, g1);
+    gs.put(GK_H, 2, g2);
 }
 
 // deterministic wave sum (butterfly)
@@ -753,8 +774,10 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             PathLDS &g = *reinterpret_cast<PathLDS *>(base);
             int ei;
             load_path(a.src, p, g, ei, n);
-            build_rotation<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_W));
-            build_rotation_d<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_Wd));
+            if constexpr (SET < FM_SET_SS) {
+                build_rotation<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_W));
+                build_rotation_d<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_Wd));
+            }
             if (lane == 0) {
                 pidx[j] = p;
                 pval[j] = valid ? 1 : 0;
@@ -769,12 +792,15 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             const int e = e0 + (il < nb ? il : nb - 1);
             if constexpr (SAVED) {
                 if (il < nb) ccj[e] = reinterpret_cast<const cx<T> *>(a.saved)[(size_t)pidx[jl] * a.saved_rec + e];
+            } else if constexpr (SET >= FM_SET_SS) {
+                const cx<T> c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gj, e);
+                if (il < nb) ccj[e] = c;
             } else {
                 unsigned char *wb = wg_work + (size_t)b * wpw;
                 const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
                                            {slice_rsrc(wb + a.work_off_F, a.work_off_acc - a.work_off_F), loc}};
                 const cx<T> pt = pass_A<T, SET>(ed, gj, Wj, ltab, e, n, pre);
-                const cx<T> c = pt * cfac_of<T>(ed, gj, e, n);
+                const cx<T> c = pt * cfac_at<T>(ed, gj, e, n);
                 if (il < nb) ccj[e] = c;
             }
         }
@@ -817,7 +843,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                     cbar = cbj[e];
                 }
             }
-            if constexpr (SAVED) {
+            if constexpr (SET >= FM_SET_SS) {
+                ss_pass_B<T, SET - FM_SET_SS + 1>(ed, gj, e, cbar, cce, gs);
+            } else if constexpr (SAVED) {
                 const PreSaved<T, SET> pre{reinterpret_cast<const cx<T> *>(a.saved) + (size_t)pidx[jl] * a.saved_rec + E + e, E};
                 pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, gs);
             } else {
@@ -988,7 +1016,9 @@ template <class T> hipError_t dispatch_b(BArgs &a, int set, hipStream_t st) {
     switch (set) {
         case 0: return launch_backward<T, 0>(a, st);
         case 1: return launch_backward<T, 1>(a, st);
-        case 2: return launch_backward<T, 2>(a, st);
+        case FM_SET_SS: return launch_backward<T, FM_SET_SS>(a, st);
+        case FM_SET_SS + 1: return launch_backward<T, FM_SET_SS + 1>(a, st);
+        case FM_SET_SS + 2: return launch_backward<T, FM_SET_SS + 2>(a, st);
     }
     return hipErrorInvalidValue;
 }
@@ -1001,8 +1031,8 @@ int check_edge_b(const fm_edge *e) {
 }
 
 int set_of(const fm_edge &ed, int nleg) {
-    if (ed.ilinit <= 1) return nleg == 2 ? 2 : 0;
-    return nleg == 2 ? -1 : 1;
+    if (nleg == 2) return FM_SET_SS + ed.ilinit - 1;
+    return ed.ilinit <= 1 ? 0 : 1;
 }
 
 }  // namespace
@@ -1037,7 +1067,6 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     if (workspace_bytes < fm_backward_workspace_bytes(precision, nleg, 16))
         return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nleg, 16)");
     const int set = set_of(*edge, nleg);
-    if (set < 0) return set_err(FM_EUNSUPPORTED, "two-leg paths of L and M edges are not implemented on the GPU");
     BArgs a;
     memset(&a, 0, sizeof a);
     a.ed = *edge;
@@ -1117,8 +1146,8 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     for (int b = 0; b < nbucket; ++b) {
         const int cnt = bucket_offset[b + 1] - bucket_offset[b];
         if (cnt <= 0) continue;
-        const int set = bucket_kind[b];
-        if (set < 0 || set > 2) return set_err(FM_EUNSUPPORTED, "bucket kind not implemented on the GPU");
+        const int kind = bucket_kind[b];
+        if (kind < 0 || kind > 2) return set_err(FM_EINVAL, "bucket kind out of range 0..2");
         const int eid = bucket_edge ? bucket_edge[b] : 0;
         if (eid < 0 || eid >= nedge) return set_err(FM_EINVAL, "bucket edge out of range");
         BArgs a;
@@ -1139,7 +1168,10 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         a.nk = grid->nk;
         a.nlegmax = bucket_nleg[b];
         if (a.nlegmax < 2 || a.nlegmax > paths->nmax) return set_err(FM_EINVAL, "bucket nleg out of range 2..nmax");
-        if ((set == 2) != (a.nlegmax == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
+        if ((kind == FM_BUCKET_SS) != (a.nlegmax == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
+        if ((kind == FM_BUCKET_K_MS) != (a.ed.ilinit <= 1) && kind != FM_BUCKET_SS)
+            return set_err(FM_EINVAL, "bucket kind does not match its edge");
+        const int set = set_of(a.ed, a.nlegmax);
         a.work = (unsigned char *)workspace;
         a.work_bytes = workspace_bytes;
         a.saved = saved;

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
     const int ltab = ed.L1 + 1, E = ed.E;
     const int nl = a.nleg;
     const size_t off_W = align16(sizeof(PathLDS));
-    const size_t off_amp = off_W + align16(sizeof(T) * rot_elems(nl, ltab));
+    const size_t off_amp = off_W + (SET < FM_SET_SS ? align16(sizeof(T) * rot_elems(nl, ltab)) : 0);
     const size_t off_ang = off_amp + align16(sizeof(T) * (size_t)E);
     const int jl = lane / cap, il = lane % cap;      // this lane's path and energy within a band
     unsigned char *lbase = smem + (size_t)jl * a.path_bytes;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             PathLDS &g = *reinterpret_cast<PathLDS *>(base);
             int ei;
             load_path(a.src, p, g, ei, nl);
-            build_rotation<T, SET>(ed, g, nl, ltab, reinterpret_cast<T *>(base + off_W));
+            if constexpr (SET < FM_SET_SS) build_rotation<T, SET>(ed, g, nl, ltab, reinterpret_cast<T *>(base + off_W));
             if (lane == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
         }
         __syncthreads();
@@ -107,7 +107,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             const bool active = il < nb && pval[jl];
             const int e = e0 + (il < nb ? il : nb - 1);
             cx<T> *rec = SAVE ? reinterpret_cast<cx<T> *>(a.saved) + (size_t)pidx[jl] * a.saved_rec : nullptr;
-            const cx<T> c = path_cchi<T, SET, SAVE>(ed, gl, Wl, ltab, e, nl, active ? rec + E + e : nullptr, E);
+            cx<T> c;
+            if constexpr (SET >= FM_SET_SS) c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gl, e);   // saves cchi only
+            else c = path_cchi<T, SET, SAVE>(ed, gl, Wl, ltab, e, nl, active ? rec + E + e : nullptr, E);
             if (SAVE && active) rec[e] = c;
             if (active) {
                 if (a.src.mode == 0) {
@@ -170,16 +172,16 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine(fm_edge ed, int n
 
 // ------------------------------------------------------------------------------------------------ launch helpers
 int set_for(const fm_edge &ed, int nleg) {
-    if (ed.ilinit <= 1) return nleg == 2 ? 2 : 0;
-    return nleg == 2 ? -1 : 1;
+    if (nleg == 2) return FM_SET_SS + ed.ilinit - 1;
+    return ed.ilinit <= 1 ? 0 : 1;
 }
 
 // Paths per workgroup: the largest plan within the LDS budget (by default room for two workgroups per CU), else the
 // largest that fits one workgroup.
 template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
-    a.path_bytes = align16(sizeof(PathLDS)) + align16(sizeof(T) * rot_elems(a.nleg, ed.L1 + 1)) + align16(sizeof(T) * ed.E) +
-                   align16(sizeof(double) * ed.E);
+    a.path_bytes = align16(sizeof(PathLDS)) + (SET < FM_SET_SS ? align16(sizeof(T) * rot_elems(a.nleg, ed.L1 + 1)) : 0) +
+                   align16(sizeof(T) * ed.E) + align16(sizeof(double) * ed.E);
     int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 0 : 2) - 256);
     if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
     if (g < 0) return hipErrorInvalidValue;
@@ -204,7 +206,9 @@ template <class T> hipError_t dispatch(KArgs &a, int set, hipStream_t st) {
     switch (set) {
         case 0: return launch_forward<T, 0>(a, st);
         case 1: return launch_forward<T, 1>(a, st);
-        case 2: return launch_forward<T, 2>(a, st);
+        case FM_SET_SS: return launch_forward<T, FM_SET_SS>(a, st);
+        case FM_SET_SS + 1: return launch_forward<T, FM_SET_SS + 1>(a, st);
+        case FM_SET_SS + 2: return launch_forward<T, FM_SET_SS + 2>(a, st);
     }
     return hipErrorInvalidValue;
 }
@@ -239,7 +243,6 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
     if (!coords || !ipots || !cchi) return set_err(FM_EINVAL, "NULL buffer");
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     const int set = set_for(*edge, nleg);
-    if (set < 0) return set_err(FM_EUNSUPPORTED, "two-leg paths of L and M edges are not implemented on the GPU");
     KArgs a;
     memset(&a, 0, sizeof a);
     a.ed = *edge;
@@ -307,8 +310,8 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
     for (int b = 0; b < nbucket; ++b) {
         const int cnt = bucket_offset[b + 1] - bucket_offset[b];
         if (cnt <= 0) continue;
-        const int set = bucket_set[b];
-        if (set < 0 || set > 2) return set_err(FM_EUNSUPPORTED, "bucket kind not implemented on the GPU");
+        const int kind = bucket_set[b];
+        if (kind < 0 || kind > 2) return set_err(FM_EINVAL, "bucket kind out of range 0..2");
         KArgs a;
         memset(&a, 0, sizeof a);
         a.src.mode = 1;
@@ -326,7 +329,10 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         a.ed = edges[a.edge_id];
         a.nleg = bucket_nleg[b];
         if (a.nleg < 2 || a.nleg > paths->nmax) return set_err(FM_EINVAL, "bucket nleg out of range 2..nmax");
-        if ((set == 2) != (a.nleg == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
+        if ((kind == FM_BUCKET_SS) != (a.nleg == 2)) return set_err(FM_EINVAL, "bucket kind does not match its nleg");
+        if ((kind == FM_BUCKET_K_MS) != (a.ed.ilinit <= 1) && kind != FM_BUCKET_SS)
+            return set_err(FM_EINVAL, "bucket kind does not match its edge");
+        const int set = set_for(a.ed, a.nleg);
         hipError_t e = precision == 32 ? dispatch<float>(a, set, st) : dispatch<double>(a, set, st);
         if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_chi_paths: the bucket's paths do not fit one workgroup's LDS");
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths launch");

@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "fm_device.h"
+#include "fm_ss.h"
 
 namespace fm {
 
@@ -304,6 +305,21 @@ __device__ __forceinline__ void save_f(cx<T> *fs, int stride, int s, const cx<T>
     for (int k = 0; k < FM_NREP; ++k) fs[((size_t)(s - 1) * FM_NREP + k) * stride] = Ft[k];
 }
 
+// cfac = exp(i R (ck - xk)) / prod_j (ck r_j), R = sum r_j = 2 reff (src/feffgpu/stage.py:336-337); the phase
+// R Re(ck - xk) in FP64 reduced mod 2 pi
+template <class T>
+__device__ __forceinline__ cx<T> cfac_at(const fm_edge &ed, const PathLDS &g, int ee, int n) {
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    double phs = g.R * ed.ckmxk[2 * ee];
+    phs -= FM_TWO_PI * rint(phs / FM_TWO_PI);
+    const double damp = exp(-g.R * ed.ckmxk[2 * ee + 1]);
+    T sp, cp;
+    if constexpr (sizeof(T) == 4) sincosf((float)phs, &sp, &cp); else sincos(phs, &sp, &cp);
+    cx<T> ckn = ck;
+    for (int j = 1; j < n; ++j) ckn = ckn * ck;
+    return cdiv(mk<T>((T)damp * cp, (T)damp * sp), ckn * (T)g.prod_r);
+}
+
 // cchi at energy e of the path held in g (tables in W). All lanes of the wave run it (e may be out of range for
 // some; they compute garbage that the caller drops, which keeps the loops uniform). With fs set, every F_s is also
 // stored (save_f).
@@ -437,23 +453,20 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
                 }
         }
     }
-    // cfac = exp(i R (ck - xk)) / prod_j (ck r_j), R = sum r_j = 2 reff (src/feffgpu/stage.py:336-337); the phase
-    // R Re(ck - xk) in FP64 reduced mod 2 pi
-    const double dkr = ed.ckmxk[2 * ee], dki = ed.ckmxk[2 * ee + 1];
-    double phs = g.R * dkr;
-    phs -= FM_TWO_PI * rint(phs / FM_TWO_PI);
-    const double damp = exp(-g.R * dki);
-    T sp, cp;
-    if constexpr (sizeof(T) == 4) {
-        sincosf((float)phs, &sp, &cp);
-    } else {
-        sincos(phs, &sp, &cp);
-    }
-    cx<T> num = mk<T>((T)damp * cp, (T)damp * sp);
-    cx<T> ckn = ck;
-    for (int j = 1; j < n; ++j) ckn = ckn * ck;
-    cx<T> den = ckn * (T)g.prod_r;
-    return cdiv(ptrac * num, den);
+    return ptrac * cfac_at<T>(ed, g, ee, n);
+}
+
+// cchi at energy ee of a single-scattering path (fm_ss.h) of an edge with initial-state il = ILI
+template <class T, int ILI>
+__device__ cx<T> ss_cchi(const fm_edge &ed, const PathLDS &g, int ee) {
+    const int E = ed.E;
+    const int lxa = max(ed.lmax[(size_t)g.ipot[0] * E + ee], ed.lmax[(size_t)g.ipot[1] * E + ee]) + 1;
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    const cx<T> z = minus_i_over(ck * (T)g.r[1]);
+    cx<T> Tk[SS_MAXR], Fk[SS_MAXR];
+    ss_reps<T, ILI, cx<T>>(ed, ee, g.ipot[1], z, lxa, Tk, Fk);
+    const cx<T> ptrac = ss_trace<T, ILI, cx<T>>(g.eta, Tk, Fk);
+    return ptrac * cfac_at<T>(ed, g, ee, 2);
 }
 
 // Fine-grid epilogue for the path held by the wave: amp/phase of cchi (lanes = energies, already in LDS as

@@ -444,6 +444,10 @@ int fm_chi_paths_plain(const fm_edge *edges, int32_t nedge, const fm_paths *path
         if (eid < 0 || eid >= nedge) return set_err(FM_EINVAL, "bucket edge out of range");
         if (nleg < 2 || nleg > paths->nmax || (set == 2) != (nleg == 2)) return set_err(FM_EINVAL, "bucket nleg out of range");
         const fm_edge &ed = edges[eid];
+        if ((set == 0 || set == 2) != (ed.ilinit <= 1))
+            return set_err(set == 2 ? FM_EUNSUPPORTED : FM_EINVAL,
+                           set == 2 ? "the plain pipeline runs two-leg paths of K and L1 edges only"
+                                    : "bucket kind does not match its edge");
         LamRt lam = lam_of(set);
         lam.mnp1 = min(ed.L1, lam.mnp1);
         // chunk: as many paths as the workspace holds

@@ -51,15 +51,12 @@ class Buckets:
 
 
 def bucket_paths(nleg: np.ndarray, edge: np.ndarray, ilinit: list) -> Buckets:
-    """Group the paths by kernel variant -- (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (K/L1 edge, 2 legs) --, by
+    """Group the paths by kernel variant -- (K/L1 edge, 3+ legs), (L/M edge, 3+ legs), (any edge, 2 legs) --, by
     edge and by number of legs: the kernels run the legs of a launch with wave-uniform loops."""
     nleg = np.asarray(nleg)
     edge = np.asarray(edge)
     il = np.asarray(ilinit)[edge]
-    kind = np.where(nleg == 2, np.where(il <= 1, N.FM_BUCKET_K_SS, -1),
-                    np.where(il <= 1, N.FM_BUCKET_K_MS, N.FM_BUCKET_LM_MS))
-    if (kind < 0).any():
-        raise NotImplementedError("two-leg paths of L and M edges are not implemented on the GPU")
+    kind = np.where(nleg == 2, N.FM_BUCKET_SS, np.where(il <= 1, N.FM_BUCKET_K_MS, N.FM_BUCKET_LM_MS))
     order = np.lexsort((np.arange(len(nleg)), -nleg, edge, kind)).astype(np.int32)
     key = ((kind * 16 + edge) * 16 + (15 - nleg))[order]
     off, kinds, edges, nlm = [0], [], [], []

@@ -126,14 +126,14 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
  *   chi_seg[s][k] = sum over the paths of segment s, in the order seg_paths lists them, of deg[p] chi_path[p][k].
  * order [npath] (device) is the processing order, a permutation of 0..npath-1 grouped into nbucket buckets:
  * bucket b holds order[bucket_offset[b] .. bucket_offset[b+1]) (host array), all of kind bucket_kind[b] (host):
- * FM_BUCKET_K_MS (K/L1 edge, 3..9 legs), FM_BUCKET_LM_MS (L2/L3/M edges, 3..9 legs), FM_BUCKET_K_SS (K/L1
+ * FM_BUCKET_K_MS (K/L1 edge, 3..9 legs), FM_BUCKET_LM_MS (L2/L3/M edges, 3..9 legs), FM_BUCKET_SS (any
  * edge, 2 legs); every path of bucket b has edge bucket_edge[b] (host, may be NULL: edge 0) and bucket_nleg[b]
  * legs (host, 2..nmax; a path whose nleg differs gets NaN for its chi_path row and reff).
  * seg_ptr [nseg + 1] and seg_paths [npath] (device) list the paths of each segment (CSR). chi_path (float for
  * precision 32, double for 64) is required: it is also the workspace of the deterministic segment sum.
  * chi_seg (double) and reff (double, A) may be NULL. saved (may be NULL, saved_bytes >= fm_saved_bytes) receives
  * the intermediates the adjoint can reuse instead of recomputing the amplitude (see fm_saved_bytes). */
-enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_K_SS = 2 };
+enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_SS = 2 };
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
                  int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,
                  const int32_t *bucket_kind, const int32_t *bucket_edge, const int32_t *bucket_nleg, int32_t nseg,

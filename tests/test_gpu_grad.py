@@ -48,7 +48,7 @@ def _cchi_loss_grad_fd(orc, coords, ipots, w, wr):
 
 
 @pytest.mark.parametrize("case,nlegs", [("validation/L001", (2, 3, 4, 5, 6)), ("validation/A018", (2, 3, 4)),
-                                        ("validation/A014", (3, 4))])
+                                        ("validation/A014", (2, 3, 4)), ("validation/A009", (2, 3))])
 def test_cchi_adjoint_matches_fd_and_fp32(case, nlegs):
     from feffgpu_amd.stage import GPUPathStage
     phase, paths = _paths(case, nlegs, 3, 1)
@@ -112,21 +112,24 @@ def test_autograd_through_cchi_and_chi_fine():
             assert err < tol, (dtype, err)
 
 
-def _box(seed, absorbers, rmax=4.6):
+def _box(seed, absorbers, rmax=4.6, npot=2):
     pos0, box = W.lattice("bcc", 2.87, 4)
     pos = W.displaced(pos0, 0.07, seed)
-    ipot = np.full(len(pos), 2)
+    ipot = np.full(len(pos), npot)
     ipot[::4] = 1
     pl = W.concat([W.enumerate_paths(pos, a, ipot, rmax, 4, box=box, segment=i, nmax=4)
                    for i, a in enumerate(absorbers)], 4)
     return pos, box, pl
 
 
-def test_chi_paths_gradient_fd_and_precisions():
+@pytest.mark.parametrize("case,npot", [("A018", 2), ("A014", 1), ("A009", 2)], ids=["K", "L3", "M5"])
+def test_chi_paths_gradient_fd_and_precisions(case, npot):
+    """Path lists of 2-4 legs on a K, an L3 and an M5 edge (two-leg paths run the single-scattering kernels)."""
     from feffgpu_amd.paths import PathPlan
-    phase = GOLDEN / "validation" / "A018" / "phase.pad"
+    phase = GOLDEN / "validation" / case / "phase.pad"
     q = np.arange(0.0, 18.0001, 0.05)
-    pos, box, pl = _box(8, [0, 33])
+    pos, box, pl = _box(8, [0, 33], npot=npot)
+    assert {2, 3, 4} <= set(pl.nleg.tolist())
     rng = np.random.default_rng(9)
     Wt = rng.normal(size=(2, len(q))) * q ** 2
     s2 = np.full(len(pl), 0.003)
@@ -152,7 +155,7 @@ def test_chi_paths_gradient_fd_and_precisions():
         an = float((grads[64] * d).sum())
         assert abs(an - fd) < 1e-6 * np.abs(grads[64]).sum() + 1e-9 * abs(fd), (an, fd)
     err32 = np.abs(grads[32] - grads[64]).max() / np.abs(grads[64]).max()
-    print(f"[grad] chi_paths: FP32 vs FP64 gradient {err32:.2e}", flush=True)
+    print(f"[grad] chi_paths {case}: FP32 vs FP64 gradient {err32:.2e}", flush=True)
     assert err32 < 1e-4
     # a periodic box has no net force from a translation of every atom
     assert np.abs(grads[64].sum(0)).max() < 1e-9 * np.abs(grads[64]).sum()

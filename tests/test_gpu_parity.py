@@ -21,10 +21,6 @@ def _stage(phase, dtype):
     return GPUPathStage(str(phase), device="cuda", dtype=dtype)
 
 
-def _supported(stage, nleg):
-    return not (stage.ilinit > 1 and nleg == 2)
-
-
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("case", CASES)
 def test_cchi_matches_oracle(case, dtype):
@@ -33,8 +29,6 @@ def test_cchi_matches_oracle(case, dtype):
     stage = _stage(phase, dtype)
     worst = 0.0
     for nleg, grp in sorted(group_by_nleg(paths).items()):
-        if not _supported(stage, nleg):
-            continue
         print(f"[cchi] {case} {dtype} nleg={nleg} paths={len(grp)}", flush=True)
         coords = torch.tensor(np.stack([p.coords for p in grp]), device="cuda", dtype=torch.float64)
         ipots = torch.tensor(np.stack([p.ipots for p in grp]), device="cuda")
@@ -58,11 +52,7 @@ def test_chi_sum_matches_feff_and_oracle(case, dtype):
     stage = _stage(phase, dtype)
     acc = np.zeros(len(k))
     acc_orc = np.zeros(len(k))
-    complete = True
     for nleg, grp in sorted(group_by_nleg(paths).items()):
-        if not _supported(stage, nleg):
-            complete = False
-            continue
         coords = torch.tensor(np.stack([p.coords for p in grp]), device="cuda", dtype=torch.float64)
         ipots = torch.tensor(np.stack([p.ipots for p in grp]), device="cuda")
         deg = np.array([p.deg for p in grp])
@@ -75,12 +65,11 @@ def test_chi_sum_matches_feff_and_oracle(case, dtype):
     rmax = rmax_of(case)
     R_orc = r_factor_vs(k, acc_orc, acc, rmax)
     rel = np.abs(acc - acc_orc).max() / np.abs(acc_orc).max()
-    print(f"[sum] {case} {dtype} complete={complete} R(vs oracle)={R_orc:.3e} maxrel={rel:.3e}", flush=True)
+    print(f"[sum] {case} {dtype} R(vs oracle)={R_orc:.3e} maxrel={rel:.3e}", flush=True)
     if dtype == torch.complex128:
         assert R_orc < 1e-14 and rel < 1e-9
     else:
         assert R_orc < 2e-5 and rel < 1e-5
-    if complete:
-        R_feff = r_factor_vs(k, chi_ref, acc, rmax)
-        print(f"[sum] {case} {dtype} R(vs FEFF chi.dat)={R_feff:.3e}", flush=True)
-        assert R_feff < (1e-9 if dtype == torch.complex128 else 2e-5)
+    R_feff = r_factor_vs(k, chi_ref, acc, rmax)
+    print(f"[sum] {case} {dtype} R(vs FEFF chi.dat)={R_feff:.3e}", flush=True)
+    assert R_feff < (1e-9 if dtype == torch.complex128 else 2e-5)

@@ -48,6 +48,41 @@ def test_chi_paths_matches_oracle_periodic(prec, tol):
     np.testing.assert_allclose(reff, ref_reff, rtol=1e-12)
 
 
+@pytest.mark.parametrize("case,npot", [("A014", 1), ("A009", 2)], ids=["L3", "M5"])
+@pytest.mark.parametrize("prec,tol", [(64, 1e-10), (32, 1e-5)], ids=["fp64", "fp32"])
+def test_chi_paths_l_and_m_edges(case, npot, prec, tol):
+    """L3 and M5 edges: multiple scattering (lambda0 = 6) and single scattering (the 15-entry lambda set, M
+    truncated at 15 entries as FEFF's setlam) against the oracle; the plain FP64 pipeline has no such two-leg
+    kernel and says so."""
+    from feffgpu_amd.paths import PathPlan
+    phase = GOLDEN / "validation" / case / "phase.pad"
+    pos0, box = W.lattice("bcc", 2.87, 4)
+    pos = W.displaced(pos0, 0.07, 21)
+    ipot = np.full(len(pos), npot)
+    ipot[::5] = 1
+    pl = W.concat([W.enumerate_paths(pos, a, ipot, 4.8, 4, box=box, segment=i, nmax=4)
+                   for i, a in enumerate([2, 50])], 4)
+    assert {2, 3, 4} <= set(pl.nleg.tolist())
+    print(f"[paths] {case} edge: {len(pl)} paths, prec {prec}", flush=True)
+    s2 = np.linspace(0.0, 0.004, len(pl))
+    ref_seg, ref_path, _ = oracle.chi_paths(oracle.OraclePhase(phase), pos, pl.atoms, pl.nleg, pl.absorber, pl.ipots,
+                                            pl.deg, Q, segment=pl.segment, nseg=2, sigma2=s2, cell=box[0],
+                                            want_paths=True)
+    plan = PathPlan(str(phase), Q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, segment=pl.segment, nseg=2,
+                    sigma2=s2, cell=box[0], precision=prec, device="cuda")
+    seg, path, _ = plan.forward(torch.tensor(pos, device="cuda"))
+    seg, path = seg.cpu().numpy(), path.double().cpu().numpy()
+    err_seg = np.abs(seg - ref_seg).max() / np.abs(ref_seg).max()
+    rel = np.abs(path - ref_path).max(1) / np.abs(ref_path).max(1)
+    for n in (2, 3, 4):
+        print(f"[paths] {case} nleg {n}: per path {rel[pl.nleg == n].max():.2e}", flush=True)
+    print(f"[paths] {case} segments {err_seg:.2e}", flush=True)
+    assert err_seg < tol and rel.max() < 5 * tol
+    if prec == 64:
+        with pytest.raises(NotImplementedError):
+            plan.forward_plain(torch.tensor(pos, device="cuda"))
+
+
 def test_general_cell_equals_cubic():
     pos, box, pl = _box_paths(4, [3, 9], rmax=4.6)
     plan_c = _plan(pl, 64, nseg=2, cell=box[0])
