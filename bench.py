@@ -73,6 +73,24 @@ def timed(fn, steps, warmup, sync):
     return (time.perf_counter() - t0) / steps
 
 
+def timed_replays(torch, step, steps, warmup, stream):
+    """`steps` replays of a captured step, bracketed by device synchronisations (wall clock), each replay also
+    between two HIP events on the replay stream. Returns (wall ms per replay, median event ms, all event ms)."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    per = [a.elapsed_time(b) for a, b in ev]
+    return wall, float(np.median(per)), per
+
+
 def cpu_baseline(pos, pl, q, cell, seconds):
     """The FP64 oracle (OpenMP over paths) on a bounded sample of the same path list."""
     import oracle
@@ -123,7 +141,7 @@ def copy_only(torch, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="md")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -133,6 +151,9 @@ def main():
                          "size, for calibrating counters) --steps times, for rocprofv3 --pmc passes")
     ap.add_argument("--lds-kib", type=int, nargs=2, default=None, metavar=("FWD", "BWD"),
                     help="launch policy (LDS budget per workgroup) instead of tuning")
+    ap.add_argument("--adjoint", choices=["recompute", "saved"], default="recompute",
+                    help="the adjoint of the timed step: amplitudes recomputed inside the backward kernel (default) or "
+                         "read from the forward's saved intermediates (both are reported)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -154,8 +175,10 @@ def main():
     print(f"[bench] workload {args.workload}: {len(pl)} paths, {len(q)} k-points, setup {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
     common = dict(segment=pl.segment, nseg=nseg, cell=cell, device=dev)
-    plan32 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32,
-                      save_intermediates=True, **common)
+    plan32r = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
+    plan32s = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32,
+                       save_intermediates=True, **common)
+    plan32 = plan32s if args.adjoint == "saved" else plan32r
     plan64 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, **common)
     pos_t = torch.tensor(pos, device=dev, dtype=torch.float64)
     sync = torch.cuda.synchronize
@@ -193,24 +216,36 @@ def main():
     Rs = [r_factor(q, s64[i], s32[i], kmin=3.0, kmax=17.0, rmin=1.0, rmax=5.0) for i in range(nseg)]
     maxrel = float(np.abs(s32 - s64).max() / np.abs(s64).max())
 
-    ms_fwd32 = timed(lambda: plan32.forward(pos_t), args.steps, args.warmup, sync) * 1e3
+    ms_fwd32 = timed(lambda: plan32r.forward(pos_t), args.steps, args.warmup, sync) * 1e3   # keeps no intermediates
     print(f"[bench] fp32 fused forward {ms_fwd32:.3f} ms/step", file=sys.stderr, flush=True)
     # forward + backward: the k^2-weighted L2 loss against a target spectrum per frame, sum_s,q (q^2 (chi - t))^2,
-    # whose gradient dL/dchi_seg = 2 q^4 (chi - t) is formed on the GPU between the two kernels
+    # whose gradient dL/dchi_seg = 2 q^4 (chi - t) is formed on the GPU between the two kernels; the whole step
+    # (bucket kernels, segment sum, loss gradient, adjoint kernels, per-atom sum) captured as one hipGraph
+    from feffgpu_amd.graph import CapturedStep
     target = torch.tensor(s64 * 0.97, device=dev)
     q4 = torch.tensor(q ** 4, device=dev)
 
-    def fwd_bwd():
-        seg, _, _ = plan32.forward(pos_t)
-        gseg = 2.0 * q4 * (seg - target)
-        return plan32.backward(pos_t, gseg)
-    ms_fb32 = timed(fwd_bwd, args.steps, args.warmup, sync) * 1e3
-    print(f"[bench] fp32 fused forward+backward {ms_fb32:.3f} ms/step", file=sys.stderr, flush=True)
-    # gradient parity of this workload: FP32 against FP64 adjoint
-    g32 = fwd_bwd().cpu().numpy()
+    def loss_grad(seg):
+        return 2.0 * q4 * (seg - target)
+    stream = torch.cuda.current_stream(dev)
+    fb = {}
+    for mode, pl_ in (("recompute", plan32r), ("saved", plan32s)):
+        step = CapturedStep(pl_, len(pos), loss_grad=loss_grad)
+        step.pos.copy_(pos_t)
+        wall, med, per = timed_replays(torch, step, args.steps, args.warmup, stream)
+        fb[mode] = dict(ms_per_step=wall, median_event_ms=med, min_event_ms=float(min(per)), graph=step)
+        print(f"[bench] fp32 fused forward+backward ({mode} adjoint, hipGraph replay) {wall:.3f} ms/step, "
+              f"median {med:.3f} ms", file=sys.stderr, flush=True)
+    ms_fb32 = fb[args.adjoint]["ms_per_step"]
+    # gradient parity of this workload: FP32 against FP64 adjoint (both adjoint modes)
+    g32 = fb[args.adjoint]["graph"]()[1].cpu().numpy()
+    g32_other = fb["saved" if args.adjoint == "recompute" else "recompute"]["graph"]()[1].cpu().numpy()
+    for m in fb.values():
+        del m["graph"]
     seg64 = plan64.forward(pos_t)[0]
     g64 = plan64.backward(pos_t, 2.0 * q4 * (seg64 - target)).cpu().numpy()
     grad_rel = float(np.abs(g32 - g64).max() / np.abs(g64).max())
+    grad_rel_other = float(np.abs(g32_other - g64).max() / np.abs(g64).max())
     ms_fwd64 = timed(lambda: plan64.forward(pos_t), max(2, args.steps // 2), 1, sync) * 1e3
     print(f"[bench] fp64 fused forward {ms_fwd64:.3f} ms/step", file=sys.stderr, flush=True)
     # the comparator: the plain FP64 pipeline (the reference's op sequence, one kernel per stage, HBM intermediates)
@@ -222,8 +257,9 @@ def main():
 
     # roofline of the dominant kernel of the step, the adjoint of the bucket with the most work (algorithmic flops
     # = 2 x the forward's, recomputation not counted), and of that bucket's forward; HIP events on the launch stream
-    stream = torch.cuda.current_stream(dev)
     nrep = max(3, args.steps)
+    if args.adjoint == "saved":
+        plan32.forward(pos_t)          # the intermediates the adjoint of the timed step reads
 
     def launch_ms(fn):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -236,7 +272,7 @@ def main():
         sync()
         return ev[0].elapsed_time(ev[1]) / nrep
 
-    kind_names = {0: "K multiple scattering", 1: "L/M multiple scattering", 2: "K single scattering"}
+    kind_names = {0: "K multiple scattering", 1: "L/M multiple scattering", 2: "single scattering"}
     what = f"{kind_names.get(int(plan32.buckets.kind[bdom]))}, {int(plan32.buckets.nleg[bdom])} legs"
     t_bwd = launch_ms(lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
     t_fwd = launch_ms(lambda: plan32.forward_bucket(pos_t, bdom))
@@ -256,13 +292,20 @@ def main():
                warmup=args.warmup, ms_per_step=ms_fb32, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="fp32 (fp64 geometry and phase)", data="synthetic",
                config=dict(cfg, paths=len(pl), parallelism="1 GPU"),
-               mode="fused FP32 forward + adjoint (k^2-weighted L2 loss per frame -> dL/dpositions)",
+               mode=f"fused FP32 forward + adjoint ({args.adjoint} amplitudes; k^2-weighted L2 loss per frame -> "
+                    "dL/dpositions), one hipGraph per step",
+               timing=f"{args.steps} graph replays between device synchronisations (wall clock); median of the "
+                      "per-replay HIP events beside it",
+               median_event_ms_per_step=fb[args.adjoint]["median_event_ms"],
+               fwd_bwd_by_adjoint=fb,
                fp32_fused_fwd=dict(value=units / (ms_fwd32 * 1e-3), ms_per_step=ms_fwd32),
                fp64_fused_fwd=dict(value=units / (ms_fwd64 * 1e-3), ms_per_step=ms_fwd64),
                fp64_plain_fwd=dict(value=units / (ms_plain64 * 1e-3), ms_per_step=ms_plain64,
                                    max_rel_vs_fused_fp64=plain_rel),
                speedup_fp32_fused_fwd_bwd_vs_fp64_plain_fwd=ms_plain64 / ms_fb32,
-               parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
+               ratio_fp64_fused_over_fp32_fused_fwd=ms_fwd64 / ms_fwd32,
+               parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel,
+                                   grad_max_rel_other_adjoint=grad_rel_other),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
                                   backward_ms=policy["backward"]),
                roofline=roof, roofline_forward=roof_fwd)

@@ -56,6 +56,8 @@ struct BArgs {
     int nslot, ltab, nk;
     int nlegmax;                // the leg count of every path of this launch
     size_t path_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_red;
+    size_t off_qs;              // 3 nk doubles of staging for the fine-grid adjoint, or 0 (none)
+    size_t off_gs;              // 3 n^2 doubles of scratch for the geometry adjoint
     const void *saved;          // intermediates of the forward call ([npath][saved_rec] complex), or NULL
     size_t saved_rec;
     unsigned char *work;        // per band slot: prefix products and scattering matrices (unless saved), gradient slots
@@ -69,7 +71,10 @@ int hip_err(hipError_t e, const char *where) {
     return fm_set_err(FM_EHIP, buf);
 }
 
-template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bool saved) {
+constexpr size_t FM_QS_MAX = 32 * 1024;   // the fine-grid adjoint stages its q points in LDS up to this size
+
+// LDS of one path. fused: the rotation tables are dead when the fine-grid adjoint runs, so its staging reuses them.
+template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bool saved, bool fused = false) {
     size_t o = align16(sizeof(PathLDS));
     const size_t tab = SET >= FM_SET_SS ? 0 : align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
     a.off_W = o; o += tab;
@@ -79,51 +84,22 @@ template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bo
     a.off_ang = o; o += align16(sizeof(double) * ed.E);
     a.off_cb = o; o += align16(sizeof(cx<T>) * ed.E);
     a.off_red = o; o += align16(sizeof(double) * (nacc(a.nlegmax) + 6 * FM_MAX_LEGS));
+    const size_t qs = sizeof(double) * 3 * (size_t)a.nk;
+    a.off_qs = 0;
+    if (a.src.mode == 1 && qs > 0 && qs <= FM_QS_MAX) {
+        if (fused && 2 * tab >= qs) a.off_qs = a.off_W;
+        else { a.off_qs = o; o += align16(qs); }
+    }
+    // the rotation tables are dead when the geometry adjoint runs
+    const size_t gsb = sizeof(double) * 3 * (size_t)a.nlegmax * a.nlegmax;
+    if (2 * tab >= gsb) a.off_gs = a.off_W;
+    else { a.off_gs = o; o += align16(gsb); }
     a.path_bytes = o;
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
     const bool store = !saved && SET < FM_SET_SS;   // single scattering keeps nothing but the gradient slots
     a.work_off_F = store ? sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE : 0;
     a.work_off_acc = store ? a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE : 0;
     a.work_per_wave = (a.work_off_acc + sizeof(T) * (size_t)nacc(a.nlegmax) * FM_WAVE + 255) & ~size_t(255);
-}
-
-// Derivative tables Wd[s][il][k] = d/dbeta d^{il-1}_{m1 m2}(beta_{s+1}) times the same normalisation as W: the
-// three-term recursion differentiated, d'^{l} = (P c - Q) d'^{l-1} - P sin(beta) d^{l-1} - R d'^{l-2}. The last leg
-// up to il = 4, as W.
-template <class T, int SET>
-__device__ __forceinline__ void build_rotation_d(const fm_edge &ed, const PathLDS &g, int n, int ltab, T *Wd) {
-    constexpr PairDef PD = pairdef(SET);
-    const int lane = lane_id();
-    for (int t = lane; t < n * FM_NREP; t += FM_WAVE) {
-        const int s = t / FM_NREP, k = t % FM_NREP;
-        int m1 = 0, m2 = 0;
-#pragma unroll
-        for (int q = 0; q < FM_NREP; ++q)
-            if (q == k) { m1 = PD.m1[q]; m2 = PD.m2[q]; }
-        const double be = g.beta[s + 1];
-        const double c = cos(be), sn = sin(be), c2 = cos(0.5 * be), s2 = sin(0.5 * be);
-        T *Ws = Wd + (size_t)s * ltab * FM_NREP;
-        const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
-        const int mm = (m1 + 3) * 7 + (m2 + 3), nn = am1 * 4 + am2;
-        const int rows = s == n - 1 ? (ed.L1 < 4 ? ed.L1 : 4) : ed.L1;
-        double dlm1 = 0.0, dl = 0.0, tlm1 = 0.0, tl = 0.0;
-        for (int il = 1; il <= rows; ++il) {
-            const int l = il - 1;
-            double d, t1;
-            if (l <= 2) {
-                d = dseed(l, m1, m2, c, sn, c2, s2);
-                t1 = dseed_db(l, m1, m2, c, sn, c2, s2);
-            } else {
-                const double *co = ed.rotc + ((size_t)(l - 1) * 49 + mm) * 3;
-                d = (co[0] * c - co[1]) * dl - co[2] * dlm1;
-                t1 = (co[0] * c - co[1]) * tl - co[0] * sn * dl - co[2] * tlm1;
-            }
-            dlm1 = dl; dl = d;
-            tlm1 = tl; tl = t1;
-            Ws[(size_t)il * FM_NREP + k] = (T)(t1 * ed.rotn[(size_t)il * 16 + nn]);
-        }
-    }
-    wave_sync();
 }
 
 // One partial-wave row of the adjoint of a block: the contractions of the row's tangents with K added to gX (gam
@@ -611,6 +587,319 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
     }
 }
 
+// ------------------------------------------------------------------------------------------------ fused adjoint
+// The adjoint of paths of NL legs in one sweep per energy, with nothing of the amplitude leaving the registers.
+// ptrac is a polynomial in the entries of the scattering matrices, so for every real parameter theta of the
+// geometry (leg lengths, angles, eta) the sweep forms the complex derivative D_theta = d ptrac / d theta without
+// the upstream gradient; the loss gradient is Re(pbc D_theta) with pbc = conj(dL/dptrac), applied once the
+// fine-grid adjoint of the whole path is known. Per energy: the F_s representatives of every scatterer (kept),
+// the termination with its tangents against Pre_n = F_{n-1} ... F_1, then s = n-1 .. 1: K_s from Pre_s (rebuilt
+// from the kept F) and U_s = T F_{n-1} ... F_{s+1}, the eta derivative, U_{s-1}, and the tangent stream of F_s.
+
+// One partial-wave row of a block's tangents contracted with K, accumulated as complex derivatives.
+template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void bwd_row_c(const cx<T> (&K)[FM_NREP], const cx<T> (&xa)[3], const cx<T> (&xd)[3],
+                                          cx<T> y0, cx<T> y1, cx<T> y0d, cx<T> y1d, const T *__restrict__ w,
+                                          const T *__restrict__ wd, cx<T> &dX, cx<T> &dY, cx<T> &dB) {
+    constexpr PairDef PD = pairdef(SET);
+    cx<T> Z[3][2], Q[3][2], Qd[3][2];
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa) {
+        Z[aa][0] = xa[aa] * y0;
+        Z[aa][1] = xa[aa] * y1;
+        Q[aa][0] = Q[aa][1] = Qd[aa][0] = Qd[aa][1] = mk<T>(T(0), T(0));
+    }
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k)
+        if (PD.ri[k] < NR && PD.rj[k] < NC) {
+            const int aa = PD.a[k] - 1, bb = PD.b[k] - 1;
+            rfma(Q[aa][bb], K[k], w[k]);
+            rfma(Qd[aa][bb], K[k], wd[k]);
+        }
+#pragma unroll
+    for (int aa = 0; aa < 3; ++aa) {
+        cfma(dB, Qd[aa][0], Z[aa][0]);
+        cfma(dB, Qd[aa][1], Z[aa][1]);
+        cfma(dX, xd[aa], Q[aa][0] * y0 + Q[aa][1] * y1);
+    }
+    cfma(dY, y0d, Q[0][0] * xa[0] + Q[1][0] * xa[1] + Q[2][0] * xa[2]);
+    cfma(dY, y1d, Q[0][1] * xa[0] + Q[1][1] * xa[1] + Q[2][1] * xa[2]);
+}
+
+// The tangent stream of one scattering matrix (rows il = 1..ilx, t-matrix one row ahead), complex derivatives.
+template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
+                                             const T *__restrict__ Wl, const T *__restrict__ Wdl,
+                                             const T *__restrict__ tl, int tlstride, cx<T> &dX, cx<T> &dY, cx<T> &dB) {
+    dX = dY = dB = mk<T>(T(0), T(0));
+    Hank3T<T> X;
+    Hank2T<T> Y;
+    X.init(minus_i_over(ck * rX), rX);
+    Y.init(minus_i_over(ck * rY), rY);
+    cx<T> xa[3], xd[3], ya[2], yd[2];
+    cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
+    const int rseed = ilx < 4 ? ilx : 4;
+    for (int r = 1; r <= rseed; ++r) {
+        X.seed_row(r, xa, xd);
+        if (r < 4) Y.seed_row(r, ya, yd); else Y.step(r, ya, yd);
+        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
+        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
+                                  Wdl + (size_t)r * FM_NREP, dX, dY, dB);
+        t = tn;
+    }
+#pragma unroll 1
+    for (int r = 5; r <= ilx; ++r) {
+        X.step(r, xa, xd);
+        Y.step(r, ya, yd);
+        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
+        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
+                                  Wdl + (size_t)r * FM_NREP, dX, dY, dB);
+        t = tn;
+    }
+}
+
+// Complex derivative slots of a lane: (kind, leg) as GSlots.
+template <class T> struct DSlots {
+    LaneArray<cx<T>> v;
+    int nleg;
+    __device__ __forceinline__ void put(int kind, int j, cx<T> x) const { v.st(kind * nleg + j - 1, x); }
+    __device__ __forceinline__ cx<T> get(int slot) const { return v.ld(slot); }
+};
+
+// F(i, j) of a block from its representatives, with the row phase: ph[i] sg Ft[idx[i][j]]
+template <class T, int SET>
+__device__ __forceinline__ cx<T> fent(const cx<T> (&Ft)[FM_NREP], const cx<T> (&ph)[6], int i, int j) {
+    constexpr PairDef PD = pairdef(SET);
+    return ph[i] * Ft[PD.idx[i][j]] * T(PD.sg[i][j]);
+}
+
+// Pre_S = F_{S-1} ... F_1 (L x L0, with the row phases) from the kept representatives; S >= 2.
+template <class T, int SET, int NL, int S>
+__device__ __forceinline__ void prefix(const PathLDS &g, const cx<T> (&F)[NL - 1][FM_NREP],
+                                       cx<T> (&P)[lamdef(SET).L][lamdef(SET).L0]) {
+    constexpr int L = lamdef(SET).L, L0 = lamdef(SET).L0;
+    cx<T> ph[6];
+    row_phases<T, SET>(ph, g, 2);
+#pragma unroll
+    for (int i = 0; i < L; ++i)
+#pragma unroll
+        for (int b = 0; b < L0; ++b) P[i][b] = fent<T, SET>(F[0], ph, i, b);
+#pragma unroll
+    for (int t = 2; t < S; ++t) {   // Pre_{t+1} = F_t Pre_t
+        row_phases<T, SET>(ph, g, t + 1);
+#pragma unroll
+        for (int b = 0; b < L0; ++b) {
+            cx<T> col[L];
+#pragma unroll
+            for (int j = 0; j < L; ++j) col[j] = P[j][b];
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                cx<T> acc = mk<T>(T(0), T(0));
+#pragma unroll
+                for (int j = 0; j < L; ++j) cfma(acc, fent<T, SET>(F[t - 1], ph, i, j), col[j]);
+                P[i][b] = acc;
+            }
+        }
+    }
+}
+
+// One chain step of the fused sweep (scatterer S; NR x NC block): K and the eta derivative from Pre_S and U, then
+// U <- U F_S (unless S = 1), then the tangent stream of F_S.
+template <class T, int SET, int NL, int S>
+__device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee,
+                                           const cx<T> (&F)[NL - 1][FM_NREP],
+                                           cx<T> (&U)[lamdef(SET).L0][lamdef(SET).L], DSlots<T> ds) {
+    constexpr LamDef D = lamdef(SET);
+    constexpr PairDef PD = pairdef(SET);
+    constexpr int L = D.L, L0 = D.L0;
+    constexpr int NR = S == NL - 1 ? L0 : L, NC = S == 1 ? L0 : L;
+    const int E = ed.E;
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    cx<T> ph[6];
+    row_phases<T, SET>(ph, g, S + 1);
+    cx<T> K[FM_NREP];
+    {
+        cx<T> Kp[FM_NREP];
+        cx<T> P[L][L0];
+        if constexpr (S >= 2) prefix<T, SET, NL, S>(g, F, P);
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) K[k] = Kp[k] = mk<T>(T(0), T(0));
+#pragma unroll
+        for (int l2 = 0; l2 < NC; ++l2) {
+#pragma unroll
+            for (int l1 = 0; l1 < NR; ++l1) {
+                cx<T> m = mk<T>(T(0), T(0));
+                if constexpr (S >= 2) {
+#pragma unroll
+                    for (int a = 0; a < L0; ++a) cfma(m, P[l2][a], U[a][l1]);
+                } else {
+                    m = U[l2][l1];
+                }
+                const cx<T> h = m * ph[l1];
+                if (PD.sg[l1][l2] > 0) K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] + h;
+                else K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] - h;
+                if (D.M[l1] != 0) rfma(Kp[PD.idx[l1][l2]], h, T(PD.sg[l1][l2] * D.M[l1]));
+            }
+        }
+        // eta_{S+1}: -i sum_k Kp[k] F_S[k]
+        cx<T> de = mk<T>(T(0), T(0));
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k)
+            if (PD.ri[k] < NR && PD.rj[k] < NC) cfma(de, Kp[k], F[S - 1][k]);
+        ds.put(GK_H, S + 1, mk<T>(de.i, -de.r));
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int a = 0; a < L0; ++a) {
+                cx<T> row[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) row[i] = (i < NR) ? U[a][i] * ph[i] : mk<T>(T(0), T(0));
+#pragma unroll
+                for (int j = 0; j < L; ++j) {
+                    cx<T> v = mk<T>(T(0), T(0));
+                    if (j < NC) {
+#pragma unroll
+                        for (int i = 0; i < NR; ++i) {
+                            const cx<T> f = F[S - 1][PD.idx[i][j]];
+                            if (PD.sg[i][j] > 0) cfma(v, row[i], f); else cfma(v, row[i], mk<T>(-f.r, -f.i));
+                        }
+                    }
+                    U[a][j] = v;
+                }
+            }
+        }
+    }
+    const int ip = g.ipot[S];
+    const int ilx = ed.lmax[(size_t)ip * E + ee] + 1;
+    const T *tl = TL<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
+    cx<T> dX, dY, dB;
+    fblock_bwd_c<T, SET, NR, NC>(K, ck, (T)g.r[S + 1], (T)g.r[S], ilx, W + (size_t)(S - 1) * ltab * FM_NREP,
+                                 Wd + (size_t)(S - 1) * ltab * FM_NREP, tl, 2 * E, dX, dY, dB);
+    ds.put(GK_X, S + 1, dX);
+    ds.put(GK_Y, S, dY);
+    ds.put(GK_B, S, dB);
+}
+
+template <class T, int SET, int NL, int S>
+__device__ __forceinline__ void fused_chain(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee,
+                                            const cx<T> (&F)[NL - 1][FM_NREP],
+                                            cx<T> (&U)[lamdef(SET).L0][lamdef(SET).L], DSlots<T> ds) {
+    fused_step<T, SET, NL, S>(ed, g, W, Wd, ltab, ee, F, U, ds);
+    if constexpr (S > 1) fused_chain<T, SET, NL, S - 1>(ed, g, W, Wd, ltab, ee, F, U, ds);
+}
+
+template <class T, int SET, int NL, int S>
+__device__ __forceinline__ void fused_fblocks(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int ee,
+                                              cx<T> (&F)[NL - 1][FM_NREP]) {
+    constexpr int L = lamdef(SET).L, L0 = lamdef(SET).L0;
+    const int E = ed.E;
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    const int ip = g.ipot[S];
+    const int ilx = ed.lmax[(size_t)ip * E + ee] + 1;
+    const T *tl = TL<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
+    const T *Wl = W + (size_t)(S - 1) * ltab * FM_NREP;
+    if constexpr (S == 1) fblock<T, SET, L, L0>(F[0], ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, 2 * E);
+    else if constexpr (S == NL - 1) fblock<T, SET, L0, L>(F[S - 1], ck, (T)g.r[S + 1], (T)g.r[S], ilx, Wl, tl, 2 * E);
+    else fblock<T, SET, L, L>(F[S - 1], ck, (T)g.r[S + 1], (T)g.r[S], ilx, Wl, tl, 2 * E);
+    if constexpr (S < NL - 1) fused_fblocks<T, SET, NL, S + 1>(ed, g, W, ltab, ee, F);
+}
+
+// The fused sweep at energy ee: returns cchi and stores the complex derivatives of ptrac in ds.
+template <class T, int SET, int NL>
+__device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab,
+                                              int ee, DSlots<T> ds) {
+    constexpr LamDef D = lamdef(SET);
+    constexpr PairDef PD = pairdef(SET);
+    constexpr int L = D.L, L0 = D.L0;
+    const int E = ed.E;
+    cx<T> F[NL - 1][FM_NREP];
+    fused_fblocks<T, SET, NL, 1>(ed, g, W, ltab, ee, F);
+    // Pre_n (L0 x L0)
+    cx<T> Pn[L0][L0];
+    {
+        cx<T> P[L][L0];
+        prefix<T, SET, NL, NL - 1>(g, F, P);
+        cx<T> ph[6];
+        row_phases<T, SET>(ph, g, NL);
+#pragma unroll
+        for (int i = 0; i < L0; ++i)
+#pragma unroll
+            for (int b = 0; b < L0; ++b) {
+                cx<T> acc = mk<T>(T(0), T(0));
+#pragma unroll
+                for (int j = 0; j < L; ++j) cfma(acc, fent<T, SET>(F[NL - 2], ph, i, j), P[j][b]);
+                Pn[i][b] = acc;
+            }
+    }
+    // termination: T with its tangents against K_T[a][b] = Pn[b][a] ph1[a]
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    cx<T> U[L0][L];
+    cx<T> ptrac = mk<T>(T(0), T(0));
+    {
+        cx<T> ph[6];
+        row_phases<T, SET>(ph, g, 1);
+        cx<T> K[FM_NREP];
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) K[k] = mk<T>(T(0), T(0));
+#pragma unroll
+        for (int a = 0; a < L0; ++a)
+#pragma unroll
+            for (int b = 0; b < L0; ++b) {
+                const cx<T> h = Pn[b][a] * ph[a];
+                if (PD.sg[a][b] > 0) K[PD.idx[a][b]] = K[PD.idx[a][b]] + h;
+                else K[PD.idx[a][b]] = K[PD.idx[a][b]] - h;
+            }
+        const int *lm = ed.lmax + ee;
+        const int lx1 = max(lm[(size_t)g.ipot[0] * E], lm[(size_t)g.ipot[1] * E]) + 1;
+        const int lxn = max(lm[(size_t)g.ipot[NL - 1] * E], lm[(size_t)g.ipot[NL] * E]) + 1;
+        Hank3T<T> X;
+        Hank2T<T> Y;
+        X.init(minus_i_over(ck * (T)g.r[1]), (T)g.r[1]);
+        Y.init(minus_i_over(ck * (T)g.r[NL]), (T)g.r[NL]);
+        cx<T> Tt[FM_NREP];
+#pragma unroll
+        for (int k = 0; k < FM_NREP; ++k) Tt[k] = mk<T>(T(0), T(0));
+        const T *Wn = W + (size_t)(NL - 1) * ltab * FM_NREP;
+        const T *Wdn = Wd + (size_t)(NL - 1) * ltab * FM_NREP;
+        const T *chw = TL<T>::chw(ed);
+        cx<T> dX = mk<T>(T(0), T(0)), dY = dX, dB = dX;
+        cx<T> xa[3], xd[3], ya[2], yd[2];
+        for (int r = 1; r <= 4; ++r) {
+            X.seed_row(r, xa, xd, r <= lx1);
+            if (r < 4) Y.seed_row(r, ya, yd, r <= lxn); else Y.step(r, ya, yd, r <= lxn);
+            int c = -1;
+            for (int q = 0; q < ed.nchan; ++q)
+                if (ed.chan_il[q] == r) c = q;
+            if (c >= 0) {
+                const cx<T> w = mk<T>(chw[((size_t)c * E + ee) * 2], chw[((size_t)c * E + ee) * 2 + 1]) * T(2 * r - 1);
+                accum_row<T, SET, L0, L0>(Tt, xa, ya[0] * w, ya[1] * w, Wn + (size_t)r * FM_NREP);
+                bwd_row_c<T, SET, L0, L0>(K, xa, xd, ya[0] * w, ya[1] * w, yd[0] * w, yd[1] * w,
+                                          Wn + (size_t)r * FM_NREP, Wdn + (size_t)r * FM_NREP, dX, dY, dB);
+            }
+        }
+        ds.put(GK_X, 1, dX);
+        ds.put(GK_Y, NL, dY);
+        ds.put(GK_B, NL, dB);
+        cx<T> de = mk<T>(T(0), T(0));
+#pragma unroll
+        for (int a = 0; a < L0; ++a)
+#pragma unroll
+            for (int b = 0; b < L; ++b) {
+                const cx<T> t = (b < L0) ? ph[a] * Tt[PD.idx[a][b]] * T(PD.sg[a][b]) : mk<T>(T(0), T(0));
+                U[a][b] = t;
+                if (b < L0) {
+                    const cx<T> tp = t * Pn[b][a];
+                    ptrac = ptrac + tp;
+                    if (D.M[a] != 0) rfma(de, tp, T(D.M[a]));
+                }
+            }
+        ds.put(GK_H, 1, mk<T>(de.i, -de.r));
+    }
+    fused_chain<T, SET, NL, NL - 1>(ed, g, W, Wd, ltab, ee, F, U, ds);
+    return ptrac * cfac_at<T>(ed, g, ee, NL);
+}
+
 // Pass B of a single-scattering path (fm_ss.h) at energy ee: the amplitude with dual numbers in the common leg
 // length r. Its derivative goes to the leg-1 slot (both legs join the same two atoms, so d r_1 = d r_2 for every
 // displacement); the angles are pi for every position (zero slots for beta); the eta slots as for any path.
@@ -643,6 +932,43 @@ __device__ __forceinline__ void ss_pass_B(const fm_edge &ed, const PathLDS &g, i
     ss_trace_eta<T, ILI, dcx<T>>(g.eta, Tk, Fk, pbc, g1, g2);
     gs.put(GK_H, 1, g1);
     gs.put(GK_H, 2, g2);
+}
+
+// The geometry adjoint of one path (one wavefront): for direction lane < ndir (atom lane / 3, plus 1 for a path list
+// whose absorber is atom 0; component lane % 3), the sum over the legs J of gp . d(r_J, beta_J, gamma_J, alpha_{J+1})
+// / d(direction) by dual numbers through the leg geometry, in 1/A. The (direction, leg) terms are spread over the
+// lanes (legs that do not touch the atom contribute 0 and are skipped) and summed over J in order; scr holds 3 n^2
+// doubles.
+__device__ __forceinline__ double geometry_adjoint(const PathLDS &g, int n, int mode, const double *gp, double *scr) {
+    const int lane = lane_id();
+    const int ndir = mode == 0 ? 3 * n : 3 * (n - 1);
+    for (int t = lane; t < ndir * n; t += FM_WAVE) {
+        const int dir = t / n, J = t - dir * n + 1;
+        const int atom = mode == 0 ? dir / 3 : dir / 3 + 1, comp = dir % 3;
+        const int i = J == n ? 0 : J, ip1 = J == n ? 1 : J + 1, im1 = J - 1;
+        const int ip1a = ip1 == n ? 0 : ip1;
+        double v = 0.0;
+        if (atom == im1 || atom == i || atom == ip1a) {
+            Dd pm[3], pi[3], pp[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pm[c] = dd(g.rat[im1][c], (im1 == atom && c == comp) ? 1.0 : 0.0);
+                pi[c] = dd(g.rat[i][c], (i == atom && c == comp) ? 1.0 : 0.0);
+                pp[c] = dd(g.rat[ip1][c], (ip1a == atom && c == comp) ? 1.0 : 0.0);
+            }
+            const LegGeoD lg = leg_geometry_dual(pm, pi, pp);
+            const int next = J == n ? 1 : J + 1;
+            v = gp[J - 1] * lg.r.d + gp[n + J - 1] * lg.beta.d + gp[2 * n + J - 1] * lg.gamma.d +
+                gp[2 * n + next - 1] * lg.alpha.d;
+        }
+        scr[t] = v;
+    }
+    wave_sync();
+    double acc = 0.0;
+    if (lane < ndir)
+        for (int J = 0; J < n; ++J) acc += scr[lane * n + J];
+    wave_sync();
+    return acc / FM_BOHR;
 }
 
 // deterministic wave sum (butterfly)
@@ -688,30 +1014,67 @@ __device__ __forceinline__ void amp_phase(int E, const cx<T> *cc, T *amp, double
 
 // Adjoint of the fine grid of one path (one wave): chi(q) = A(q) sin(Phi(q) + 2 q reff) exp(-2 sigma2 q^2), A and
 // Phi linear in amp[] and the unwrapped ang[]. From dL/dchi(q) = gup_scale gup_d[q] + gup_p[q] to dL/dcchi per
-// energy: each energy gathers the q points that interpolate from it (qorder / qstart runs, in a fixed order), and
-// dL/dA(q), dL/dPhi(q) are evaluated where they are used. Returns dL/dreff (A), wave-summed.
+// energy: each energy gathers the q points that interpolate from it (qorder / qstart runs, in a fixed order).
+// With qs (3 nk doubles of LDS), dL/dA(q), dL/dPhi(q) and the fraction are first evaluated once per q point with
+// the lanes over the points (their loads independent), then gathered from LDS; without it, each energy evaluates
+// them where it uses them. Returns dL/dreff (A), wave-summed.
 template <class T>
 __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, const double *fq, const double *q, const int32_t *qorder,
                                     const int32_t *qstart, double reff_A, double sigma2, const double *gup_d,
                                     double gup_scale, const T *gup_p, const cx<T> *cc, const T *amp, const double *ang,
-                                    cx<T> *cb) {
+                                    cx<T> *cb, int nk = 0, double *qs = nullptr) {
     const int lane = lane_id();
-    auto adj = [&](int i, double &gA, double &gP) {
-        const int j = jq[i];
-        const double f = fq[i];
+    // dL/dA and dL/dPhi at q point i from its interpolation (j, f), q and the upstream gradient gu
+    auto eval = [&](int j, double f, double qi, double gu, double &gA, double &gP) {
         const double A = (1.0 - f) * (double)amp[j] + f * (double)amp[j + 1];
-        double ph = (1.0 - f) * ang[j] + f * ang[j + 1] + 2.0 * q[i] * reff_A;
+        double ph = (1.0 - f) * ang[j] + f * ang[j + 1] + 2.0 * qi * reff_A;
         ph -= FM_TWO_PI * rint(ph / FM_TWO_PI);
-        double gu = 0.0;
-        if (gup_d) gu += gup_scale * gup_d[i];
-        if (gup_p) gu += (double)gup_p[i];
-        if (sigma2 != 0.0) gu *= exp(-2.0 * sigma2 * q[i] * q[i]);
+        if (sigma2 != 0.0) gu *= exp(-2.0 * sigma2 * qi * qi);
         T sn, cs;
         if constexpr (sizeof(T) == 4) sincosf((float)ph, &sn, &cs); else sincos(ph, &sn, &cs);
         gA = (double)(T)(gu * (double)sn);
         gP = (double)(T)(gu * A * (double)cs);
     };
+    auto gup = [&](int i) -> double {
+        double gu = 0.0;
+        if (gup_d) gu += gup_scale * gup_d[i];
+        if (gup_p) gu += (double)gup_p[i];
+        return gu;
+    };
+    auto adj = [&](int i, double &gA, double &gP) { eval(jq[i], fq[i], q[i], gup(i), gA, gP); };
     double rb = 0.0;
+    if (qs) {
+        double *sA = qs, *sP = qs + nk, *sF = qs + 2 * nk;
+        const int t1 = qstart[E];
+        for (int t = lane; t < t1; t += FM_WAVE) {
+            const int i = qorder[t];
+            double gA, gP;
+            adj(i, gA, gP);
+            sA[t] = gA;
+            sP[t] = gP;
+            sF[t] = fq[i];
+            rb += 2.0 * q[i] * gP;
+        }
+        wave_sync();
+        for (int e = lane; e < E; e += FM_WAVE) {
+            double ab = 0.0, pb = 0.0;
+            const int lo0 = qstart[e], lo1 = qstart[e + 1], hi0 = e > 0 ? qstart[e - 1] : lo0;
+            for (int t = lo0; t < lo1; ++t) {
+                ab += (1.0 - sF[t]) * sA[t];
+                pb += (1.0 - sF[t]) * sP[t];
+            }
+            for (int t = hi0; t < lo0; ++t) {
+                ab += sF[t] * sA[t];
+                pb += sF[t] * sP[t];
+            }
+            const double cr = (double)cc[e].r, ci = (double)cc[e].i;
+            const double c2 = cr * cr + ci * ci, a = (double)amp[e];
+            cx<T> v = mk<T>(T(0), T(0));
+            if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
+            cb[e] = v;
+        }
+        return wave_sum(rb);
+    }
     for (int e = lane; e < E; e += FM_WAVE) {
         double ab = 0.0, pb = 0.0;
         for (int t = qstart[e]; t < qstart[e + 1]; ++t) {    // q points with lower interpolation energy e
@@ -774,16 +1137,13 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             PathLDS &g = *reinterpret_cast<PathLDS *>(base);
             int ei;
             load_path(a.src, p, g, ei, n);
-            if constexpr (SET < FM_SET_SS) {
-                build_rotation<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_W));
-                build_rotation_d<T, SET>(ed, g, n, ltab, reinterpret_cast<T *>(base + a.off_Wd));
-            }
             if (lane == 0) {
                 pidx[j] = p;
                 pval[j] = valid ? 1 : 0;
                 rbar[j] = (a.src.mode == 0 && a.greff) ? a.greff[p] : 0.0;
             }
         }
+        if constexpr (SET < FM_SET_SS) build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, n, ltab);
         __syncthreads();
         // ---- 2. pass A over this wavefront's bands (or the forward call's saved cchi)
         for (int b = 0; b < nband; ++b) {
@@ -822,7 +1182,8 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                 const double rb = fine_grid_adjoint<T>(E, a.G.jq + (size_t)a.edge_id * a.nk, a.G.fq + (size_t)a.edge_id * a.nk,
                                                        a.G.q, a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride,
                                                        g.reff * FM_BOHR, a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr,
-                                                       a.src.P.deg[p], gpr, cc, amp, ang, cb);
+                                                       a.src.P.deg[p], gpr, cc, amp, ang, cb, a.nk,
+                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr);
                 if (lane == 0) rbar[j] = rb;
             }
         __syncthreads();
@@ -871,7 +1232,14 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                 for (int b = 0; b < nband; ++b) {
                     const int nb = a.bp.start[b + 1] - a.bp.start[b];
                     const T *src = reinterpret_cast<const T *>(wg_work + (size_t)b * wpw + a.work_off_acc) + (size_t)lane * FM_WAVE + j * cap;
-                    for (int i = 0; i < nb; ++i) s += (double)src[i];
+                    for (int i0 = 0; i0 < nb; i0 += 8) {   // 8 loads in flight, summed in order
+                        T d[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) d[u] = i0 + u < nb ? src[i0 + u] : T(0);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (i0 + u < nb) s += (double)d[u];
+                    }
                 }
                 red[lane] = s;
             }
@@ -890,29 +1258,175 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                 gp[lane] = v;
             }
             wave_sync();
-            // geometry adjoint: one lane per coordinate direction (bohr), dual numbers through every leg
+            // geometry adjoint (dual numbers through the legs)
             const int ndir = a.src.mode == 0 ? 3 * n : 3 * (n - 1);
-            double gout = 0.0;
+            double gout = geometry_adjoint(g, n, a.src.mode, gp, reinterpret_cast<double *>(base + a.off_gs));
             if (lane < ndir) {
-                const int atom = a.src.mode == 0 ? lane / 3 : lane / 3 + 1, comp = lane % 3;
-                auto point = [&](int idx, Dd (&v)[3]) {
-                    const int ai = idx == n ? 0 : idx;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[c] = dd(g.rat[idx][c], (ai == atom && c == comp) ? 1.0 : 0.0);
-                };
-                double acc_g = 0.0;
-                for (int J = 1; J <= n; ++J) {
-                    const int i = J == n ? 0 : J, ip1 = J == n ? 1 : J + 1, im1 = J - 1;
-                    Dd pm[3], pi[3], pp[3];
-                    point(im1, pm);
-                    point(i, pi);
-                    point(ip1, pp);
-                    const LegGeoD lg = leg_geometry_dual(pm, pi, pp);
-                    const int next = J == n ? 1 : J + 1;
-                    acc_g += gp[J - 1] * lg.r.d + gp[n + J - 1] * lg.beta.d + gp[2 * n + J - 1] * lg.gamma.d +
-                             gp[2 * n + next - 1] * lg.alpha.d;
+                if (!g.ok) gout = __builtin_nan("");
+                tmp[lane] = gout;
+            }
+            wave_sync();
+            const int p = pidx[j];
+            double *out = a.gcoord + (size_t)p * a.nslot * 3;
+            if (a.src.mode == 0) {
+                if (lane < 3 * n) out[lane] = gout;
+            } else {
+                if (lane < 3 * (n - 1)) out[3 + lane] = gout;
+                if (lane < 3) {
+                    double s = 0.0;
+                    for (int k = 0; k < n - 1; ++k) s += tmp[3 * k + lane];
+                    out[lane] = -s;
                 }
-                gout = acc_g / FM_BOHR;
+                if (lane >= 3 * n && lane < 3 * a.nslot) out[lane] = 0.0;
+            }
+            wave_sync();
+        }
+        __syncthreads();
+    }
+}
+
+// The fused adjoint of multiple-scattering paths of NL legs: per band of energies one sweep (fused_energy) that
+// leaves cchi in LDS and the complex derivatives of ptrac in the band's workspace; then per path the fine-grid
+// adjoint (or the given dL/dcchi), pbc and the cfac terms per energy; then the derivative slots contracted with pbc
+// and summed over the energies in band order, and the geometry adjoint.
+template <class T, int SET, int NL, int WPE>
+__global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adjoint(BArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int pidx[8], pval[8];
+    __shared__ double rbar[8];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
+    const int lane = lane_id();
+    const fm_edge &ed = a.ed;
+    const int G = a.gp, cap = FM_WAVE / G;
+    const int ltab = ed.L1 + 1, E = ed.E;
+    const int nband = a.bp.nband;
+    const uint32_t loc = (uint32_t)lane * sizeof(cx<T>);
+    const int jl = lane / cap, il = lane % cap;
+    unsigned char *pbase = smem + (size_t)jl * a.path_bytes;
+    const PathLDS &gj = *reinterpret_cast<const PathLDS *>(pbase);
+    const T *Wj = reinterpret_cast<const T *>(pbase + a.off_W);
+    const T *Wdj = reinterpret_cast<const T *>(pbase + a.off_Wd);
+    cx<T> *ccj = reinterpret_cast<cx<T> *>(pbase + a.off_cc);
+    unsigned char *const wg_work = a.work + (size_t)blockIdx.x * nband * a.work_per_wave;
+    const size_t wpw = a.work_per_wave;
+
+    for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
+        // ---- 1. setup: one wavefront per path
+        for (int j = wid; j < G; j += 4) {
+            const int slot = blk * G + j;
+            const bool valid = slot < a.npath_bucket;
+            const int s0 = valid ? slot : blk * G;
+            const int p = a.order ? a.order[a.first + s0] : a.first + s0;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
+            PathLDS &g = *reinterpret_cast<PathLDS *>(base);
+            int ei;
+            load_path(a.src, p, g, ei, NL);
+            if (lane == 0) {
+                pidx[j] = p;
+                pval[j] = valid ? 1 : 0;
+                rbar[j] = (a.src.mode == 0 && a.greff) ? a.greff[p] : 0.0;
+            }
+        }
+        build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
+        __syncthreads();
+        // ---- 2. the fused sweep over this wavefront's bands
+        for (int b = 0; b < nband; ++b) {
+            if (a.bp.wave[b] != wid) continue;
+            const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
+            const int e = e0 + (il < nb ? il : nb - 1);
+            const DSlots<T> ds{{slice_rsrc(wg_work + (size_t)b * wpw, wpw), loc}, NL};
+            const cx<T> c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds);
+            if (il < nb) ccj[e] = c;
+        }
+        __syncthreads();
+        // ---- 3. per path: dL/dcchi per energy, then pbc = conj(dL/dptrac) (in cb) and the two cfac terms (in amp, ang)
+        for (int j = wid; j < G; j += 4) {
+            if (!pval[j]) continue;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
+            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+            const cx<T> *cc = reinterpret_cast<const cx<T> *>(base + a.off_cc);
+            T *amp = reinterpret_cast<T *>(base + a.off_amp);
+            double *ang = reinterpret_cast<double *>(base + a.off_ang);
+            cx<T> *cb = reinterpret_cast<cx<T> *>(base + a.off_cb);
+            const int p = pidx[j];
+            if (a.src.mode == 1) {
+                amp_phase<T>(E, cc, amp, ang);
+                const double *gsr = a.gseg ? a.gseg + (size_t)a.segment[p] * a.nk : nullptr;
+                const T *gpr = a.gpath ? reinterpret_cast<const T *>(a.gpath) + (size_t)p * a.nk : nullptr;
+                const double rb = fine_grid_adjoint<T>(E, a.G.jq + (size_t)a.edge_id * a.nk, a.G.fq + (size_t)a.edge_id * a.nk,
+                                                       a.G.q, a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride,
+                                                       g.reff * FM_BOHR, a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr,
+                                                       a.src.P.deg[p], gpr, cc, amp, ang, cb, a.nk,
+                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr);
+                if (lane == 0) rbar[j] = rb;
+            } else {
+                const T *gin = reinterpret_cast<const T *>(a.gcchi) + (size_t)p * E * 2;
+                for (int e = lane; e < E; e += FM_WAVE) cb[e] = mk<T>(gin[2 * e], gin[2 * e + 1]);
+            }
+            wave_sync();
+            for (int e = lane; e < E; e += FM_WAVE) {
+                const cx<T> cbar = cb[e];
+                const cx<T> w = conj(cbar) * cc[e];
+                const cx<T> wdk = w * mk<T>((T)ed.ckmxk[2 * e], (T)ed.ckmxk[2 * e + 1]);
+                amp[e] = -wdk.i;                                      // d cfac / d r_j: a1 - a2 / r_j
+                ang[e] = (double)w.r;
+                cb[e] = conj(cbar * conj(cfac_at<T>(ed, g, e, NL)));
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        // ---- 4. per path: Re(pbc D) summed over the energies in band order, the geometry adjoint, the output
+        for (int j = wid; j < G; j += 4) {
+            if (!pval[j]) continue;
+            unsigned char *base = smem + (size_t)j * a.path_bytes;
+            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+            const T *amp = reinterpret_cast<const T *>(base + a.off_amp);
+            const double *ang = reinterpret_cast<const double *>(base + a.off_ang);
+            const cx<T> *pbc = reinterpret_cast<const cx<T> *>(base + a.off_cb);
+            double *red = reinterpret_cast<double *>(base + a.off_red);
+            double *gp = red + nacc(NL);
+            double *tmp = gp + 3 * FM_MAX_LEGS;
+            constexpr int n = NL;
+            if (lane < 4 * n) {
+                double s = 0.0;
+                for (int b = 0; b < nband; ++b) {
+                    const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
+                    const cx<T> *src = reinterpret_cast<const cx<T> *>(wg_work + (size_t)b * wpw) + (size_t)lane * FM_WAVE + j * cap;
+                    for (int i0 = 0; i0 < nb; i0 += 8) {   // 8 loads in flight, summed in order
+                        cx<T> d[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) d[u] = i0 + u < nb ? src[i0 + u] : mk<T>(T(0), T(0));
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (i0 + u < nb) {
+                                const cx<T> q = pbc[e0 + i0 + u];
+                                s += (double)(q.r * d[u].r - q.i * d[u].i);
+                            }
+                    }
+                }
+                red[lane] = s;
+            } else if (lane < 4 * n + 2) {
+                double s = 0.0;
+                for (int e = 0; e < E; ++e) s += lane == 4 * n ? (double)amp[e] : ang[e];
+                red[lane] = s;
+            }
+            wave_sync();
+            if (lane < 3 * n) {
+                const int kind = lane / n, jg = lane % n + 1;
+                double v;
+                if (kind == 0)
+                    v = red[GK_X * n + jg - 1] + red[GK_Y * n + jg - 1] + red[4 * n] - red[4 * n + 1] / g.r[jg] +
+                        0.5 * FM_BOHR * rbar[j];
+                else if (kind == 1)
+                    v = red[GK_B * n + jg - 1];
+                else
+                    v = red[GK_H * n + jg - 1];
+                gp[lane] = v;
+            }
+            wave_sync();
+            const int ndir = a.src.mode == 0 ? 3 * n : 3 * (n - 1);
+            double gout = geometry_adjoint(g, n, a.src.mode, gp, reinterpret_cast<double *>(base + a.off_gs));
+            if (lane < ndir) {
                 if (!g.ok) gout = __builtin_nan("");
                 tmp[lane] = gout;
             }
@@ -989,12 +1503,55 @@ template <class T, int SET, int WPE, bool SAVED> hipError_t launch_k(const BArgs
     return hipGetLastError();
 }
 
+constexpr int FM_FUSED_MAXLEG = 6;   // multiple-scattering paths up to this many legs run the fused adjoint
+
+template <class T, int SET, int NL> hipError_t launch_fused_nl(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
+    const void *fn = (const void *)k_adjoint<T, SET, NL, 1>;
+    if (shm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_adjoint<T, SET, NL, 1>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
+    return hipGetLastError();
+}
+
+// The fused adjoint (multiple scattering, 3..FM_FUSED_MAXLEG legs): the same LDS layout as k_backward, a workspace
+// slice per band holding 4 nleg complex derivative slots per lane.
+template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
+    const fm_edge &ed = a.ed;
+    layout<T, SET>(a, ed, true, true);
+    a.work_per_wave = ((size_t)4 * a.nlegmax * FM_WAVE * sizeof(cx<T>) + 255) & ~size_t(255);
+    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3) - 256);
+    if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+    if (g < 0) return hipErrorInvalidValue;
+    a.gp = 1 << g;
+    a.bp = ed.bands[g];
+    const size_t shm = a.path_bytes * a.gp;
+    const long slots = (long)(a.work_bytes / a.work_per_wave);
+    if (slots < a.bp.nband) return hipErrorInvalidValue;
+    long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
+    if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
+    if (blocks == 0) return hipSuccess;
+    switch (a.nlegmax) {
+        case 3: return launch_fused_nl<T, SET, 3>(a, blocks, shm, st);
+        case 4: return launch_fused_nl<T, SET, 4>(a, blocks, shm, st);
+        case 5: return launch_fused_nl<T, SET, 5>(a, blocks, shm, st);
+        case 6: return launch_fused_nl<T, SET, 6>(a, blocks, shm, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
     a.ltab = ed.L1 + 1;
+    if constexpr (SET < FM_SET_SS) {
+        if (a.nlegmax >= 3 && a.nlegmax <= FM_FUSED_MAXLEG) return launch_fused<T, SET>(a, st);
+    }
     const bool saved = a.saved != nullptr;
     layout<T, SET>(a, ed, saved);
-    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3) - 256);
+    size_t budget = fm_lds_budget(sizeof(T) == 4 ? 1 : 3);
+    if (SET >= FM_SET_SS && budget > FM_LDS_SHARE) budget = FM_LDS_SHARE;   // single scattering: two workgroups per CU
+    int g = pick_gpaths(ed, a.path_bytes, budget - 256);
     if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
     if (g < 0) return hipErrorInvalidValue;
     a.gp = 1 << g;

@@ -96,9 +96,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             PathLDS &g = *reinterpret_cast<PathLDS *>(base);
             int ei;
             load_path(a.src, p, g, ei, nl);
-            if constexpr (SET < FM_SET_SS) build_rotation<T, SET>(ed, g, nl, ltab, reinterpret_cast<T *>(base + off_W));
             if (lane == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
         }
+        if constexpr (SET < FM_SET_SS) build_tables<T, SET, false, 1>(ed, smem, a.path_bytes, off_W, 0, wid, 4, G, nl, ltab);
         __syncthreads();
         // ---- energies: the bands of this wavefront
         for (int b = 0; b < a.bp.nband; ++b) {

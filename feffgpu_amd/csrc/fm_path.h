@@ -179,40 +179,101 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
     return n;
 }
 
-// Rotation tables: W[s][il][k] = d^{il-1}_{m1 m2}(beta_{s+1}) xnlm(il,|m1|)(-1)^|m1| / xnlm(il,|m2|) for legs
-// s = 0..n-1 and the FM_NREP representatives k, il = 1..L1 (row 0 unused; the last leg, which only the
-// termination reads, up to il = 4). Three-term recursion in l for fixed (m1, m2) (src/feffgpu/stage.py:77-92),
-// seeded by the closed forms at l <= 2, in FP64.
-template <class T, int SET>
-__device__ void build_rotation(const fm_edge &ed, const PathLDS &g, int n, int ltab, T *W) {
+// The recursion coefficients of B consecutive rows il0.. of one (m1, m2) (mm, nn as build_tables forms them),
+// fetched together so that their load latencies overlap: from l - 1 to l for l >= 3, and the normalisation.
+template <int B> struct RotCoef {
+    double c0[B], c1[B], c2[B], nr[B];
+    __device__ __forceinline__ void fetch(const fm_edge &ed, int il0, int rows, int mm, int nn) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int il = il0 + u, l = il - 1;
+            nr[u] = il <= rows ? ed.rotn[(size_t)il * 16 + nn] : 0.0;
+            if (il <= rows && l >= 3) {
+                const double *co = ed.rotc + ((size_t)(l - 1) * 49 + mm) * 3;
+                c0[u] = co[0];
+                c1[u] = co[1];
+                c2[u] = co[2];
+            } else {
+                c0[u] = c1[u] = c2[u] = 0.0;
+            }
+        }
+    }
+};
+
+// Rotation tables of the paths j = j0, j0 + jstep, ... < G held by a wavefront (path j at smem + j path_bytes, its
+// PathLDS first): W[s][il][k] = d^{il-1}_{m1 m2}(beta_{s+1}) xnlm(il,|m1|)(-1)^|m1| / xnlm(il,|m2|) for legs s = 0..n-1
+// and the FM_NREP representatives k, il = 1..L1 (row 0 unused; the last leg, which only the termination reads, up to
+// il = 4); with WITH_D also Wd = the same with d/dbeta d. Three-term recursion in l for fixed (m1, m2)
+// (src/feffgpu/stage.py:77-92) seeded by the closed forms at l <= 2, in FP64, differentiated:
+// d'^l = (P c - Q) d'^{l-1} - P sin(beta) d^{l-1} - R d'^{l-2}. The (path, leg, representative) tasks of all the
+// wavefront's paths are spread over its lanes. The coefficients of BATCH rows are fetched at a time (the forward
+// kernel, at its register limit, takes 1).
+template <class T, int SET, bool WITH_D, int BATCH = 8, int ILP = 1>
+__device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path_bytes, size_t off_W, size_t off_Wd,
+                             int j0, int jstep, int G, int n, int ltab) {
     constexpr PairDef PD = pairdef(SET);
     const int lane = lane_id();
-    const int ntask = n * FM_NREP;
-    for (int t = lane; t < ntask; t += FM_WAVE) {
-        const int s = t / FM_NREP, k = t % FM_NREP;
-        int m1 = 0, m2 = 0;
+    const int np = j0 < G ? (G - 1 - j0) / jstep + 1 : 0;
+    const int per = n * FM_NREP, ntask = np * per;
+    // ILP independent recursions per lane (tasks t, t + 64, ...), interleaved row by row
+    for (int t0 = lane; t0 < ntask; t0 += FM_WAVE * ILP) {
+        int m1[ILP], m2[ILP], mm[ILP], nn[ILP], rows[ILP], kk[ILP];
+        double c[ILP], sn[ILP], c2[ILP], s2[ILP], dlm1[ILP], dl[ILP], tlm1[ILP], tl[ILP];
+        T *Ws[ILP], *Wds[ILP];
+        int rmax = 0;
 #pragma unroll
-        for (int q = 0; q < FM_NREP; ++q)
-            if (q == k) { m1 = PD.m1[q]; m2 = PD.m2[q]; }
-        const double be = g.beta[s + 1];
-        double c = cos(be), sn = sin(be), c2 = cos(0.5 * be), s2 = sin(0.5 * be);
-        T *Ws = W + (size_t)s * ltab * FM_NREP;
-        const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
-        const int mm = (m1 + 3) * 7 + (m2 + 3), nn = am1 * 4 + am2;
-        const int rows = s == n - 1 ? (ed.L1 < 4 ? ed.L1 : 4) : ed.L1;
-        double dlm1 = 0.0, dl = 0.0;
-        for (int il = 1; il <= rows; ++il) {
-            const int l = il - 1;
-            double d;
-            if (l <= 2) {
-                d = dseed(l, m1, m2, c, sn, c2, s2);
-            } else {
-                const double *co = ed.rotc + ((size_t)(l - 1) * 49 + mm) * 3;   // from l - 1 to l
-                d = (co[0] * c - co[1]) * dl - co[2] * dlm1;
+        for (int q = 0; q < ILP; ++q) {
+            const int t = t0 + q * FM_WAVE;
+            const bool on = t < ntask;
+            const int tt = on ? t : t0;
+            const int jj = tt / per, rr = tt - jj * per;
+            const int s = rr / FM_NREP, k = rr - s * FM_NREP;
+            unsigned char *base = smem + (size_t)(j0 + jj * jstep) * path_bytes;
+            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
+            m1[q] = m2[q] = 0;
+#pragma unroll
+            for (int r = 0; r < FM_NREP; ++r)
+                if (r == k) { m1[q] = PD.m1[r]; m2[q] = PD.m2[r]; }
+            const double be = g.beta[s + 1];
+            c[q] = cos(be); sn[q] = sin(be); c2[q] = cos(0.5 * be); s2[q] = sin(0.5 * be);
+            Ws[q] = reinterpret_cast<T *>(base + off_W) + (size_t)s * ltab * FM_NREP;
+            Wds[q] = reinterpret_cast<T *>(base + off_Wd) + (size_t)s * ltab * FM_NREP;
+            kk[q] = k;
+            const int am1 = m1[q] < 0 ? -m1[q] : m1[q], am2 = m2[q] < 0 ? -m2[q] : m2[q];
+            mm[q] = (m1[q] + 3) * 7 + (m2[q] + 3);
+            nn[q] = am1 * 4 + am2;
+            rows[q] = on ? (s == n - 1 ? (ed.L1 < 4 ? ed.L1 : 4) : ed.L1) : 0;
+            rmax = rows[q] > rmax ? rows[q] : rmax;
+            dlm1[q] = dl[q] = tlm1[q] = tl[q] = 0.0;
+        }
+        for (int il0 = 1; il0 <= rmax; il0 += BATCH) {
+            RotCoef<BATCH> rc[ILP];
+#pragma unroll
+            for (int q = 0; q < ILP; ++q) rc[q].fetch(ed, il0, rows[q], mm[q], nn[q]);
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int il = il0 + u, l = il - 1;
+                if (il > rmax) break;
+#pragma unroll
+                for (int q = 0; q < ILP; ++q) {
+                    if (il > rows[q]) continue;
+                    double d, t1 = 0.0;
+                    if (l <= 2) {
+                        d = dseed(l, m1[q], m2[q], c[q], sn[q], c2[q], s2[q]);
+                        if constexpr (WITH_D) t1 = dseed_db(l, m1[q], m2[q], c[q], sn[q], c2[q], s2[q]);
+                    } else {
+                        const double a1 = rc[q].c0[u] * c[q] - rc[q].c1[u];
+                        d = a1 * dl[q] - rc[q].c2[u] * dlm1[q];
+                        if constexpr (WITH_D) t1 = a1 * tl[q] - rc[q].c0[u] * sn[q] * dl[q] - rc[q].c2[u] * tlm1[q];
+                    }
+                    dlm1[q] = dl[q]; dl[q] = d;
+                    Ws[q][(size_t)il * FM_NREP + kk[q]] = (T)(d * rc[q].nr[u]);
+                    if constexpr (WITH_D) {
+                        tlm1[q] = tl[q]; tl[q] = t1;
+                        Wds[q][(size_t)il * FM_NREP + kk[q]] = (T)(t1 * rc[q].nr[u]);
+                    }
+                }
             }
-            dlm1 = dl;
-            dl = d;
-            Ws[(size_t)il * FM_NREP + k] = (T)(d * ed.rotn[(size_t)il * 16 + nn]);
         }
     }
     wave_sync();
