@@ -151,9 +151,6 @@ def main():
                          "size, for calibrating counters) --steps times, for rocprofv3 --pmc passes")
     ap.add_argument("--lds-kib", type=int, nargs=2, default=None, metavar=("FWD", "BWD"),
                     help="launch policy (LDS budget per workgroup) instead of tuning")
-    ap.add_argument("--adjoint", choices=["recompute", "saved"], default="recompute",
-                    help="the adjoint of the timed step: amplitudes recomputed inside the backward kernel (default) or "
-                         "read from the forward's saved intermediates (both are reported)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -175,10 +172,7 @@ def main():
     print(f"[bench] workload {args.workload}: {len(pl)} paths, {len(q)} k-points, setup {time.perf_counter() - t_setup:.1f} s",
           file=sys.stderr, flush=True)
     common = dict(segment=pl.segment, nseg=nseg, cell=cell, device=dev)
-    plan32r = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
-    plan32s = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32,
-                       save_intermediates=True, **common)
-    plan32 = plan32s if args.adjoint == "saved" else plan32r
+    plan32 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
     plan64 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, **common)
     pos_t = torch.tensor(pos, device=dev, dtype=torch.float64)
     sync = torch.cuda.synchronize
@@ -197,8 +191,6 @@ def main():
     bdom, b_work = dominant_bucket(plan32, pl)
     gseg_one = torch.ones(nseg, len(q), dtype=torch.float64, device=dev)
     if args.kernel_only:
-        if args.kernel_only == "bwd":
-            plan32.forward(pos_t)      # the intermediates the adjoint of the timed step reads
         run = (lambda: plan32.forward_bucket(pos_t, bdom)) if args.kernel_only == "fwd" else \
               (lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
         for _ in range(args.steps):
@@ -216,7 +208,7 @@ def main():
     Rs = [r_factor(q, s64[i], s32[i], kmin=3.0, kmax=17.0, rmin=1.0, rmax=5.0) for i in range(nseg)]
     maxrel = float(np.abs(s32 - s64).max() / np.abs(s64).max())
 
-    ms_fwd32 = timed(lambda: plan32r.forward(pos_t), args.steps, args.warmup, sync) * 1e3   # keeps no intermediates
+    ms_fwd32 = timed(lambda: plan32.forward(pos_t), args.steps, args.warmup, sync) * 1e3
     print(f"[bench] fp32 fused forward {ms_fwd32:.3f} ms/step", file=sys.stderr, flush=True)
     # forward + backward: the k^2-weighted L2 loss against a target spectrum per frame, sum_s,q (q^2 (chi - t))^2,
     # whose gradient dL/dchi_seg = 2 q^4 (chi - t) is formed on the GPU between the two kernels; the whole step
@@ -228,24 +220,17 @@ def main():
     def loss_grad(seg):
         return 2.0 * q4 * (seg - target)
     stream = torch.cuda.current_stream(dev)
-    fb = {}
-    for mode, pl_ in (("recompute", plan32r), ("saved", plan32s)):
-        step = CapturedStep(pl_, len(pos), loss_grad=loss_grad)
-        step.pos.copy_(pos_t)
-        wall, med, per = timed_replays(torch, step, args.steps, args.warmup, stream)
-        fb[mode] = dict(ms_per_step=wall, median_event_ms=med, min_event_ms=float(min(per)), graph=step)
-        print(f"[bench] fp32 fused forward+backward ({mode} adjoint, hipGraph replay) {wall:.3f} ms/step, "
-              f"median {med:.3f} ms", file=sys.stderr, flush=True)
-    ms_fb32 = fb[args.adjoint]["ms_per_step"]
-    # gradient parity of this workload: FP32 against FP64 adjoint (both adjoint modes)
-    g32 = fb[args.adjoint]["graph"]()[1].cpu().numpy()
-    g32_other = fb["saved" if args.adjoint == "recompute" else "recompute"]["graph"]()[1].cpu().numpy()
-    for m in fb.values():
-        del m["graph"]
+    step = CapturedStep(plan32, len(pos), loss_grad=loss_grad)
+    step.pos.copy_(pos_t)
+    ms_fb32, med_fb32, per = timed_replays(torch, step, args.steps, args.warmup, stream)
+    print(f"[bench] fp32 fused forward+backward (hipGraph replay) {ms_fb32:.3f} ms/step, median {med_fb32:.3f} ms",
+          file=sys.stderr, flush=True)
+    # gradient parity of this workload: FP32 against FP64 adjoint
+    g32 = step()[1].cpu().numpy()
+    del step
     seg64 = plan64.forward(pos_t)[0]
     g64 = plan64.backward(pos_t, 2.0 * q4 * (seg64 - target)).cpu().numpy()
     grad_rel = float(np.abs(g32 - g64).max() / np.abs(g64).max())
-    grad_rel_other = float(np.abs(g32_other - g64).max() / np.abs(g64).max())
     ms_fwd64 = timed(lambda: plan64.forward(pos_t), max(2, args.steps // 2), 1, sync) * 1e3
     print(f"[bench] fp64 fused forward {ms_fwd64:.3f} ms/step", file=sys.stderr, flush=True)
     # the comparator: the plain FP64 pipeline (the reference's op sequence, one kernel per stage, HBM intermediates)
@@ -258,8 +243,6 @@ def main():
     # roofline of the dominant kernel of the step, the adjoint of the bucket with the most work (algorithmic flops
     # = 2 x the forward's, recomputation not counted), and of that bucket's forward; HIP events on the launch stream
     nrep = max(3, args.steps)
-    if args.adjoint == "saved":
-        plan32.forward(pos_t)          # the intermediates the adjoint of the timed step reads
 
     def launch_ms(fn):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -292,20 +275,18 @@ def main():
                warmup=args.warmup, ms_per_step=ms_fb32, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="fp32 (fp64 geometry and phase)", data="synthetic",
                config=dict(cfg, paths=len(pl), parallelism="1 GPU"),
-               mode=f"fused FP32 forward + adjoint ({args.adjoint} amplitudes; k^2-weighted L2 loss per frame -> "
-                    "dL/dpositions), one hipGraph per step",
+               mode="fused FP32 forward + adjoint (amplitudes recomputed in the adjoint; k^2-weighted L2 loss per "
+                    "frame -> dL/dpositions), one hipGraph per step",
                timing=f"{args.steps} graph replays between device synchronisations (wall clock); median of the "
                       "per-replay HIP events beside it",
-               median_event_ms_per_step=fb[args.adjoint]["median_event_ms"],
-               fwd_bwd_by_adjoint=fb,
+               median_event_ms_per_step=med_fb32, min_event_ms_per_step=float(min(per)),
                fp32_fused_fwd=dict(value=units / (ms_fwd32 * 1e-3), ms_per_step=ms_fwd32),
                fp64_fused_fwd=dict(value=units / (ms_fwd64 * 1e-3), ms_per_step=ms_fwd64),
                fp64_plain_fwd=dict(value=units / (ms_plain64 * 1e-3), ms_per_step=ms_plain64,
                                    max_rel_vs_fused_fp64=plain_rel),
                speedup_fp32_fused_fwd_bwd_vs_fp64_plain_fwd=ms_plain64 / ms_fb32,
                ratio_fp64_fused_over_fp32_fused_fwd=ms_fwd64 / ms_fwd32,
-               parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel,
-                                   grad_max_rel_other_adjoint=grad_rel_other),
+               parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
                                   backward_ms=policy["backward"]),
                roofline=roof, roofline_forward=roof_fwd)

@@ -89,13 +89,11 @@ def lib():
                                            _vp, _vp, _vp]
         L.fm_chi_paths.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid), _i32,
                                    _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
-                                   ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
-        L.fm_saved_bytes.restype = ctypes.c_size_t
-        L.fm_saved_bytes.argtypes = [_i32, _i32, _i32, _i32]
+                                   ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, _vp]
         L.fm_chi_paths_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths),
                                             ctypes.POINTER(FmGrid), _i32, _i32, _vp, _i32, ctypes.POINTER(_i32),
                                             ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp,
-                                            _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, ctypes.c_size_t, _vp]
+                                            _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
         L.fm_plain_path_bytes.restype = ctypes.c_size_t
         L.fm_plain_path_bytes.argtypes = [ctypes.POINTER(FmEdge), _i32]
         L.fm_chi_paths_plain.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid),

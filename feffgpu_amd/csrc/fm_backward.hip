@@ -58,9 +58,7 @@ struct BArgs {
     size_t path_bytes, off_W, off_Wd, off_cc, off_amp, off_ang, off_cb, off_red;
     size_t off_qs;              // 3 nk doubles of staging for the fine-grid adjoint, or 0 (none)
     size_t off_gs;              // 3 n^2 doubles of scratch for the geometry adjoint
-    const void *saved;          // intermediates of the forward call ([npath][saved_rec] complex), or NULL
-    size_t saved_rec;
-    unsigned char *work;        // per band slot: prefix products and scattering matrices (unless saved), gradient slots
+    unsigned char *work;        // per band slot: prefix products and scattering matrices, gradient slots
     size_t work_per_wave, work_off_F, work_off_acc, work_bytes;
 };
 
@@ -74,7 +72,7 @@ int hip_err(hipError_t e, const char *where) {
 constexpr size_t FM_QS_MAX = 32 * 1024;   // the fine-grid adjoint stages its q points in LDS up to this size
 
 // LDS of one path. fused: the rotation tables are dead when the fine-grid adjoint runs, so its staging reuses them.
-template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bool saved, bool fused = false) {
+template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bool fused = false) {
     size_t o = align16(sizeof(PathLDS));
     const size_t tab = SET >= FM_SET_SS ? 0 : align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
     a.off_W = o; o += tab;
@@ -96,7 +94,7 @@ template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bo
     else { a.off_gs = o; o += align16(gsb); }
     a.path_bytes = o;
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
-    const bool store = !saved && SET < FM_SET_SS;   // single scattering keeps nothing but the gradient slots
+    const bool store = SET < FM_SET_SS;   // single scattering keeps nothing but the gradient slots
     a.work_off_F = store ? sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE : 0;
     a.work_off_acc = store ? a.work_off_F + sizeof(cx<T>) * (size_t)(a.nlegmax - 1) * FM_NREP * FM_WAVE : 0;
     a.work_per_wave = (a.work_off_acc + sizeof(T) * (size_t)nacc(a.nlegmax) * FM_WAVE + 255) & ~size_t(255);
@@ -243,47 +241,6 @@ template <class T, int SET> struct PreStore {
         for (int i = 0; i < L; ++i)
 #pragma unroll
             for (int b = 0; b < L0; ++b) P[i][b] = get(s - 2, i * L0 + b);
-    }
-};
-
-// The same interface over the intermediates a forward call saved (save_f): F_s of this lane at
-// f[((s - 1) * FM_NREP + k) * stride]; Pre_s = F_{s-1} ... F_1 (with the row phases) is rebuilt from them.
-template <class T, int SET> struct PreSaved {
-    const cx<T> *f;
-    int stride;
-    __device__ __forceinline__ void get_f(int s, cx<T> (&Ft)[FM_NREP]) const {
-#pragma unroll
-        for (int k = 0; k < FM_NREP; ++k) Ft[k] = f[((size_t)(s - 1) * FM_NREP + k) * stride];
-    }
-    __device__ __forceinline__ void mat(int s, const PathLDS &g, cx<T> (&P)[lamdef(SET).L][lamdef(SET).L0]) const {
-        constexpr LamDef D = lamdef(SET);
-        constexpr PairDef PD = pairdef(SET);
-        constexpr int L = D.L, L0 = D.L0;
-        cx<T> Ft[FM_NREP], ph[6];
-        get_f(1, Ft);
-        row_phases<T, SET>(ph, g, 2);
-#pragma unroll
-        for (int i = 0; i < L; ++i)
-#pragma unroll
-            for (int b = 0; b < L0; ++b) P[i][b] = ph[i] * Ft[PD.idx[i][b]] * T(PD.sg[i][b]);
-#pragma unroll 1
-        for (int t = 2; t < s; ++t) {   // Pre_{t+1} = F_t Pre_t
-            get_f(t, Ft);
-            row_phases<T, SET>(ph, g, t + 1);
-#pragma unroll
-            for (int b = 0; b < L0; ++b) {
-                cx<T> col[L];
-#pragma unroll
-                for (int j = 0; j < L; ++j) col[j] = P[j][b];
-#pragma unroll
-                for (int i = 0; i < L; ++i) {
-                    cx<T> acc = mk<T>(T(0), T(0));
-#pragma unroll
-                    for (int j = 0; j < L; ++j) cfma(acc, Ft[PD.idx[i][j]] * T(PD.sg[i][j]), col[j]);
-                    P[i][b] = ph[i] * acc;
-                }
-            }
-        }
     }
 };
 
@@ -900,40 +857,6 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
     return ptrac * cfac_at<T>(ed, g, ee, NL);
 }
 
-// Pass B of a single-scattering path (fm_ss.h) at energy ee: the amplitude with dual numbers in the common leg
-// length r. Its derivative goes to the leg-1 slot (both legs join the same two atoms, so d r_1 = d r_2 for every
-// displacement); the angles are pi for every position (zero slots for beta); the eta slots as for any path.
-template <class T, int ILI>
-__device__ __forceinline__ void ss_pass_B(const fm_edge &ed, const PathLDS &g, int ee, cx<T> cbar, cx<T> cc, GSlots<T> gs) {
-    const int E = ed.E;
-    {
-        const cx<T> w = conj(cbar) * cc;
-        const cx<T> dk = mk<T>((T)ed.ckmxk[2 * ee], (T)ed.ckmxk[2 * ee + 1]);
-        const cx<T> wdk = w * dk;
-        gs.put_slot(4 * gs.nlegmax, -wdk.i);
-        gs.put_slot(4 * gs.nlegmax + 1, w.r);
-    }
-    const cx<T> pbc = conj(cbar * conj(cfac_at<T>(ed, g, ee, 2)));
-    const int lxa = max(ed.lmax[(size_t)g.ipot[0] * E + ee], ed.lmax[(size_t)g.ipot[1] * E + ee]) + 1;
-    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
-    const T r = (T)g.r[1];
-    const cx<T> z = minus_i_over(ck * r);
-    dcx<T> Tk[SS_MAXR], Fk[SS_MAXR];
-    ss_reps<T, ILI, dcx<T>>(ed, ee, g.ipot[1], dcx<T>{z, z * (T(-1) / r)}, lxa, Tk, Fk);
-    const dcx<T> pt = ss_trace<T, ILI, dcx<T>>(g.eta, Tk, Fk);
-    const cx<T> gr = pbc * pt.d;
-    gs.put(GK_X, 1, gr.r);
-    gs.put(GK_X, 2, T(0));
-    gs.put(GK_Y, 1, T(0));
-    gs.put(GK_Y, 2, T(0));
-    gs.put(GK_B, 1, T(0));
-    gs.put(GK_B, 2, T(0));
-    T g1, g2;
-    ss_trace_eta<T, ILI, dcx<T>>(g.eta, Tk, Fk, pbc, g1, g2);
-    gs.put(GK_H, 1, g1);
-    gs.put(GK_H, 2, g2);
-}
-
 // The geometry adjoint of one path (one wavefront): for direction lane < ndir (atom lane / 3, plus 1 for a path list
 // whose absorber is atom 0; component lane % 3), the sum over the legs J of gp . d(r_J, beta_J, gamma_J, alpha_{J+1})
 // / d(direction) by dual numbers through the leg geometry, in 1/A. The (direction, leg) terms are spread over the
@@ -1102,7 +1025,7 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
     return wave_sum(rb);
 }
 
-template <class T, int SET, int WPE, bool SAVED>
+template <class T, int SET, int WPE>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_backward(BArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int pidx[8], pval[8];
@@ -1143,19 +1066,14 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                 rbar[j] = (a.src.mode == 0 && a.greff) ? a.greff[p] : 0.0;
             }
         }
-        if constexpr (SET < FM_SET_SS) build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, n, ltab);
+        build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, n, ltab);
         __syncthreads();
-        // ---- 2. pass A over this wavefront's bands (or the forward call's saved cchi)
+        // ---- 2. pass A over this wavefront's bands
         for (int b = 0; b < nband; ++b) {
             if (a.bp.wave[b] != wid) continue;
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
             const int e = e0 + (il < nb ? il : nb - 1);
-            if constexpr (SAVED) {
-                if (il < nb) ccj[e] = reinterpret_cast<const cx<T> *>(a.saved)[(size_t)pidx[jl] * a.saved_rec + e];
-            } else if constexpr (SET >= FM_SET_SS) {
-                const cx<T> c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gj, e);
-                if (il < nb) ccj[e] = c;
-            } else {
+            {
                 unsigned char *wb = wg_work + (size_t)b * wpw;
                 const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
                                            {slice_rsrc(wb + a.work_off_F, a.work_off_acc - a.work_off_F), loc}};
@@ -1204,12 +1122,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                     cbar = cbj[e];
                 }
             }
-            if constexpr (SET >= FM_SET_SS) {
-                ss_pass_B<T, SET - FM_SET_SS + 1>(ed, gj, e, cbar, cce, gs);
-            } else if constexpr (SAVED) {
-                const PreSaved<T, SET> pre{reinterpret_cast<const cx<T> *>(a.saved) + (size_t)pidx[jl] * a.saved_rec + E + e, E};
-                pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, gs);
-            } else {
+            {
                 const PreStore<T, SET> pre{{slice_rsrc(wb, a.work_off_F), loc},
                                            {slice_rsrc(wb + a.work_off_F, a.work_off_acc - a.work_off_F), loc}};
                 pass_B<T, SET, WPE>(ed, gj, Wj, Wdj, ltab, e, n, cbar, cce, pre, gs);
@@ -1285,6 +1198,33 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
     }
 }
 
+// The fused sweep of a single-scattering path (fm_ss.h) at energy ee: cchi, and the complex derivatives of ptrac
+// along the common leg length (leg-1 slot; d r_1 = d r_2 for every displacement, the angles are pi for every
+// position) and the two eta, with dual numbers in the leg length.
+template <class T, int ILI>
+__device__ __forceinline__ cx<T> ss_fused_energy(const fm_edge &ed, const PathLDS &g, int ee, DSlots<T> ds) {
+    const int E = ed.E;
+    const int lxa = max(ed.lmax[(size_t)g.ipot[0] * E + ee], ed.lmax[(size_t)g.ipot[1] * E + ee]) + 1;
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    const T r = (T)g.r[1];
+    const cx<T> z = minus_i_over(ck * r);
+    dcx<T> Tk[SS_MAXR], Fk[SS_MAXR];
+    ss_reps<T, ILI, dcx<T>>(ed, ee, g.ipot[1], dcx<T>{z, z * (T(-1) / r)}, lxa, Tk, Fk);
+    const dcx<T> pt = ss_trace<T, ILI, dcx<T>>(g.eta, Tk, Fk);
+    const cx<T> zero = mk<T>(T(0), T(0));
+    ds.put(GK_X, 1, pt.d);
+    ds.put(GK_X, 2, zero);
+    ds.put(GK_Y, 1, zero);
+    ds.put(GK_Y, 2, zero);
+    ds.put(GK_B, 1, zero);
+    ds.put(GK_B, 2, zero);
+    cx<T> h1, h2;
+    ss_trace_eta<T, ILI, dcx<T>>(g.eta, Tk, Fk, h1, h2);
+    ds.put(GK_H, 1, mk<T>(h1.i, -h1.r));
+    ds.put(GK_H, 2, mk<T>(h2.i, -h2.r));
+    return pt.v * cfac_at<T>(ed, g, ee, 2);
+}
+
 // The fused adjoint of multiple-scattering paths of NL legs: per band of energies one sweep (fused_energy) that
 // leaves cchi in LDS and the complex derivatives of ptrac in the band's workspace; then per path the fine-grid
 // adjoint (or the given dL/dcchi), pbc and the cfac terms per energy; then the derivative slots contracted with pbc
@@ -1327,7 +1267,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                 rbar[j] = (a.src.mode == 0 && a.greff) ? a.greff[p] : 0.0;
             }
         }
-        build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
+        if constexpr (SET < FM_SET_SS) build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
         __syncthreads();
         // ---- 2. the fused sweep over this wavefront's bands
         for (int b = 0; b < nband; ++b) {
@@ -1335,7 +1275,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
             const int e = e0 + (il < nb ? il : nb - 1);
             const DSlots<T> ds{{slice_rsrc(wg_work + (size_t)b * wpw, wpw), loc}, NL};
-            const cx<T> c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds);
+            cx<T> c;
+            if constexpr (SET >= FM_SET_SS) c = ss_fused_energy<T, SET - FM_SET_SS + 1>(ed, gj, e, ds);
+            else c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds);
             if (il < nb) ccj[e] = c;
         }
         __syncthreads();
@@ -1493,64 +1435,38 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, i
 // Paths per workgroup: the largest plan within the LDS budget (by default one workgroup per CU, whose 4 wavefronts
 // then have the whole register file: 1 wavefront per SIMD); a plan small enough for two workgroups per CU runs the
 // variant sized for 2 wavefronts per SIMD (FP32).
-template <class T, int SET, int WPE, bool SAVED> hipError_t launch_k(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
+template <class T, int SET, int WPE> hipError_t launch_k(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
     if (shm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_backward<T, SET, WPE, SAVED>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_backward<T, SET, WPE>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_backward<T, SET, WPE, SAVED>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
+    hipLaunchKernelGGL((k_backward<T, SET, WPE>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
 constexpr int FM_FUSED_MAXLEG = 6;   // multiple-scattering paths up to this many legs run the fused adjoint
 
-template <class T, int SET, int NL> hipError_t launch_fused_nl(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
-    const void *fn = (const void *)k_adjoint<T, SET, NL, 1>;
+template <class T, int SET, int NL, int WPE = 1>
+hipError_t launch_fused_nl(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
+    const void *fn = (const void *)k_adjoint<T, SET, NL, WPE>;
     if (shm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_adjoint<T, SET, NL, 1>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
+    hipLaunchKernelGGL((k_adjoint<T, SET, NL, WPE>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
-// The fused adjoint (multiple scattering, 3..FM_FUSED_MAXLEG legs): the same LDS layout as k_backward, a workspace
-// slice per band holding 4 nleg complex derivative slots per lane.
+// The fused adjoint (multiple scattering of 3..FM_FUSED_MAXLEG legs, single scattering): the same LDS layout as
+// k_backward, a workspace slice per band holding 4 nleg complex derivative slots per lane. Single scattering keeps
+// little in registers: two workgroups per CU (FP32).
 template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
-    layout<T, SET>(a, ed, true, true);
+    layout<T, SET>(a, ed, true);
     a.work_per_wave = ((size_t)4 * a.nlegmax * FM_WAVE * sizeof(cx<T>) + 255) & ~size_t(255);
-    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3) - 256);
-    if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
-    if (g < 0) return hipErrorInvalidValue;
-    a.gp = 1 << g;
-    a.bp = ed.bands[g];
-    const size_t shm = a.path_bytes * a.gp;
-    const long slots = (long)(a.work_bytes / a.work_per_wave);
-    if (slots < a.bp.nband) return hipErrorInvalidValue;
-    long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
-    if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
-    if (blocks == 0) return hipSuccess;
-    switch (a.nlegmax) {
-        case 3: return launch_fused_nl<T, SET, 3>(a, blocks, shm, st);
-        case 4: return launch_fused_nl<T, SET, 4>(a, blocks, shm, st);
-        case 5: return launch_fused_nl<T, SET, 5>(a, blocks, shm, st);
-        case 6: return launch_fused_nl<T, SET, 6>(a, blocks, shm, st);
-    }
-    return hipErrorInvalidValue;
-}
-
-template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st) {
-    const fm_edge &ed = a.ed;
-    a.ltab = ed.L1 + 1;
-    if constexpr (SET < FM_SET_SS) {
-        if (a.nlegmax >= 3 && a.nlegmax <= FM_FUSED_MAXLEG) return launch_fused<T, SET>(a, st);
-    }
-    const bool saved = a.saved != nullptr;
-    layout<T, SET>(a, ed, saved);
     size_t budget = fm_lds_budget(sizeof(T) == 4 ? 1 : 3);
-    if (SET >= FM_SET_SS && budget > FM_LDS_SHARE) budget = FM_LDS_SHARE;   // single scattering: two workgroups per CU
+    if (SET >= FM_SET_SS && budget > FM_LDS_SHARE) budget = FM_LDS_SHARE;
     int g = pick_gpaths(ed, a.path_bytes, budget - 256);
     if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
     if (g < 0) return hipErrorInvalidValue;
@@ -1562,11 +1478,46 @@ template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st)
     long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
     if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
     if (blocks == 0) return hipSuccess;
-    if constexpr (sizeof(T) == 4) {
-        if (shm <= FM_LDS_SHARE - 256)
-            return saved ? launch_k<T, SET, 2, true>(a, blocks, shm, st) : launch_k<T, SET, 2, false>(a, blocks, shm, st);
+    if constexpr (SET >= FM_SET_SS) {
+        if (sizeof(T) == 4 && shm <= FM_LDS_SHARE - 256) return launch_fused_nl<T, SET, 2, 2>(a, blocks, shm, st);
+        return launch_fused_nl<T, SET, 2, 1>(a, blocks, shm, st);
+    } else {
+        switch (a.nlegmax) {
+            case 3: return launch_fused_nl<T, SET, 3>(a, blocks, shm, st);
+            case 4: return launch_fused_nl<T, SET, 4>(a, blocks, shm, st);
+            case 5: return launch_fused_nl<T, SET, 5>(a, blocks, shm, st);
+            case 6: return launch_fused_nl<T, SET, 6>(a, blocks, shm, st);
+        }
+        return hipErrorInvalidValue;
     }
-    return saved ? launch_k<T, SET, 1, true>(a, blocks, shm, st) : launch_k<T, SET, 1, false>(a, blocks, shm, st);
+}
+
+// Multiple scattering of more than FM_FUSED_MAXLEG legs: pass A stores the prefix products in the workspace, pass B
+// reads them (k_backward). Everything else: the fused adjoint.
+template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st) {
+    const fm_edge &ed = a.ed;
+    a.ltab = ed.L1 + 1;
+    if constexpr (SET >= FM_SET_SS) {
+        return launch_fused<T, SET>(a, st);
+    } else {
+        if (a.nlegmax >= 3 && a.nlegmax <= FM_FUSED_MAXLEG) return launch_fused<T, SET>(a, st);
+        layout<T, SET>(a, ed);
+        int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3) - 256);
+        if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+        if (g < 0) return hipErrorInvalidValue;
+        a.gp = 1 << g;
+        a.bp = ed.bands[g];
+        const size_t shm = a.path_bytes * a.gp;
+        const long slots = (long)(a.work_bytes / a.work_per_wave);
+        if (slots < a.bp.nband) return hipErrorInvalidValue;
+        long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
+        if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
+        if (blocks == 0) return hipSuccess;
+        if constexpr (sizeof(T) == 4) {
+            if (shm <= FM_LDS_SHARE - 256) return launch_k<T, SET, 2>(a, blocks, shm, st);
+        }
+        return launch_k<T, SET, 1>(a, blocks, shm, st);
+    }
 }
 
 template <class T> hipError_t dispatch_b(BArgs &a, int set, hipStream_t st) {
@@ -1595,11 +1546,6 @@ int set_of(const fm_edge &ed, int nleg) {
 }  // namespace
 
 extern "C" {
-
-size_t fm_saved_bytes(int32_t precision, int32_t nmax, int32_t emax, int32_t npath) {
-    if (nmax < 2 || nmax > FM_MAX_LEGS || emax < 2 || emax > FM_MAX_E || npath < 0) return 0;
-    return (precision == 32 ? 8 : 16) * (size_t)(1 + (nmax - 1) * FM_NREP) * emax * (size_t)npath;
-}
 
 size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t band_slots) {
     if (nlegmax < 2 || nlegmax > FM_MAX_LEGS || band_slots < 1) return 0;
@@ -1678,7 +1624,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
                           const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          const void *saved, size_t saved_bytes, void *workspace, size_t workspace_bytes, void *stream) {
+                          void *workspace, size_t workspace_bytes, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     int emax = 0;
     for (int i = 0; i < nedge; ++i) {
@@ -1695,9 +1641,6 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     if (grid->qstride < emax + 1) return set_err(FM_EINVAL, "grid.qstride < E + 1");
     if (!workspace || workspace_bytes < fm_backward_workspace_bytes(precision, paths->nmax, 16))
         return set_err(FM_EINVAL, "workspace smaller than fm_backward_workspace_bytes(precision, nmax, 16)");
-    const size_t rec = (size_t)(1 + (paths->nmax - 1) * FM_NREP) * emax;
-    if (saved && saved_bytes < fm_saved_bytes(precision, paths->nmax, emax, paths->npath))
-        return set_err(FM_EINVAL, "saved smaller than fm_saved_bytes(precision, nmax, largest E, npath)");
     if (nbucket > 0 && (!bucket_offset || !bucket_kind || !bucket_nleg)) return set_err(FM_EINVAL, "NULL bucket arrays");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {
@@ -1731,8 +1674,6 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         const int set = set_of(a.ed, a.nlegmax);
         a.work = (unsigned char *)workspace;
         a.work_bytes = workspace_bytes;
-        a.saved = saved;
-        a.saved_rec = rec;
         hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
         if (e == hipErrorInvalidValue)
             return set_err(FM_EINVAL, "fm_chi_paths_backward: the bucket's paths do not fit one workgroup's LDS");

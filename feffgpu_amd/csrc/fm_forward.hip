@@ -51,8 +51,6 @@ struct KArgs {
     fm_grid G;            // mode 1
     void *chi_path;
     double *reff_out;
-    void *saved;          // mode 1: per path cchi[E] and F_s for the adjoint ([npath][saved_rec] complex), or NULL
-    size_t saved_rec;
     int edge_id;          // index of the edge (rows of the grid's jq / fq)
     int nleg;             // the leg count of every path of this launch
     size_t path_bytes;    // LDS per path of a workgroup
@@ -68,7 +66,7 @@ int hip_err(hipError_t e, const char *where) {
 }
 
 
-template <class T, int SET, bool SAVE>
+template <class T, int SET>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_WPE(T)))) void k_forward(KArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int pidx[8], pval[8];
@@ -106,11 +104,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
             const bool active = il < nb && pval[jl];
             const int e = e0 + (il < nb ? il : nb - 1);
-            cx<T> *rec = SAVE ? reinterpret_cast<cx<T> *>(a.saved) + (size_t)pidx[jl] * a.saved_rec : nullptr;
             cx<T> c;
-            if constexpr (SET >= FM_SET_SS) c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gl, e);   // saves cchi only
-            else c = path_cchi<T, SET, SAVE>(ed, gl, Wl, ltab, e, nl, active ? rec + E + e : nullptr, E);
-            if (SAVE && active) rec[e] = c;
+            if constexpr (SET >= FM_SET_SS) c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gl, e);
+            else c = path_cchi<T, SET>(ed, gl, Wl, ltab, e, nl);
             if (active) {
                 if (a.src.mode == 0) {
                     T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)pidx[jl] * E + e) * 2;
@@ -188,17 +184,14 @@ template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st) 
     a.gp = 1 << g;
     a.bp = ed.bands[g];
     const size_t shm = a.path_bytes * a.gp;
-    const void *fn = a.saved ? (const void *)k_forward<T, SET, true> : (const void *)k_forward<T, SET, false>;
+    const void *fn = (const void *)k_forward<T, SET>;
     if (shm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
     const int blocks = (a.npath_bucket + a.gp - 1) / a.gp;
     if (blocks == 0) return hipSuccess;
-    if (a.saved)
-        hipLaunchKernelGGL((k_forward<T, SET, true>), dim3(blocks), dim3(FM_WAVE * 4), shm, st, a);
-    else
-        hipLaunchKernelGGL((k_forward<T, SET, false>), dim3(blocks), dim3(FM_WAVE * 4), shm, st, a);
+    hipLaunchKernelGGL((k_forward<T, SET>), dim3(blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
@@ -289,7 +282,7 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
                          int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
                          const int32_t *bucket_offset, const int32_t *bucket_set, const int32_t *bucket_edge,
                          const int32_t *bucket_nleg, int32_t nseg, const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg,
-                         void *chi_path, double *reff, void *saved, size_t saved_bytes, void *stream) {
+                         void *chi_path, double *reff, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     for (int i = 0; i < nedge; ++i) {
         int rc = check_edge(&edges[i]);
@@ -301,11 +294,6 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
     if (paths->nmax < 2 || paths->nmax > FM_MAX_LEGS) return set_err(FM_EINVAL, "nmax out of range 2..9");
     if (grid->nk < 1) return set_err(FM_EINVAL, "empty grid");
     if (nbucket > 0 && (!bucket_offset || !bucket_set || !bucket_nleg)) return set_err(FM_EINVAL, "NULL bucket arrays");
-    int emax = 0;
-    for (int i = 0; i < nedge; ++i) emax = emax > edges[i].E ? emax : edges[i].E;
-    const size_t rec = (size_t)(1 + (paths->nmax - 1) * FM_NREP) * emax;
-    if (saved && saved_bytes < (precision == 32 ? 8 : 16) * rec * (size_t)paths->npath)
-        return set_err(FM_EINVAL, "saved smaller than fm_saved_bytes(precision, nmax, largest E, npath)");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {
         const int cnt = bucket_offset[b + 1] - bucket_offset[b];
@@ -322,8 +310,6 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         a.G = *grid;
         a.chi_path = chi_path;
         a.reff_out = reff;
-        a.saved = saved;
-        a.saved_rec = rec;
         a.edge_id = bucket_edge ? bucket_edge[b] : 0;
         if (a.edge_id < 0 || a.edge_id >= nedge) return set_err(FM_EINVAL, "bucket edge out of range");
         a.ed = edges[a.edge_id];

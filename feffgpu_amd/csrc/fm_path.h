@@ -357,15 +357,6 @@ template <> struct TL<double> {
     static __device__ __forceinline__ const double *chw(const fm_edge &e) { return e.chw64; }
 };
 
-// Stores the FM_NREP representatives of F_s (s = 1..n-1) at fs[((s - 1) * FM_NREP + k) * stride] when fs is set:
-// the intermediates an adjoint can reuse instead of recomputing the amplitude.
-template <class T, bool SAVE>
-__device__ __forceinline__ void save_f(cx<T> *fs, int stride, int s, const cx<T> (&Ft)[FM_NREP]) {
-    if (!SAVE || !fs) return;
-#pragma unroll
-    for (int k = 0; k < FM_NREP; ++k) fs[((size_t)(s - 1) * FM_NREP + k) * stride] = Ft[k];
-}
-
 // cfac = exp(i R (ck - xk)) / prod_j (ck r_j), R = sum r_j = 2 reff (src/feffgpu/stage.py:336-337); the phase
 // R Re(ck - xk) in FP64 reduced mod 2 pi
 template <class T>
@@ -382,11 +373,9 @@ __device__ __forceinline__ cx<T> cfac_at(const fm_edge &ed, const PathLDS &g, in
 }
 
 // cchi at energy e of the path held in g (tables in W). All lanes of the wave run it (e may be out of range for
-// some; they compute garbage that the caller drops, which keeps the loops uniform). With fs set, every F_s is also
-// stored (save_f).
-template <class T, int SET, bool SAVE = false>
-__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n, cx<T> *fs = nullptr,
-                           int fstride = 0) {
+// some; they compute garbage that the caller drops, which keeps the loops uniform).
+template <class T, int SET>
+__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
@@ -467,7 +456,6 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
         const T *tl, *Wl;
         scat(1, ilx, tl, Wl);
         fblock<T, SET, L0, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
-        save_f<T, SAVE>(fs, fstride, 1, Ft);
         row_phases<T, SET>(ph, g, 2);
 #pragma unroll
         for (int a = 0; a < L0; ++a)
@@ -483,7 +471,6 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(n - 1, ilx, tl, Wl);
             fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
-            save_f<T, SAVE>(fs, fstride, n - 1, Ft);
             row_phases<T, SET>(ph, g, n);
             chain(std::integral_constant<int, L0>{});
         }
@@ -493,7 +480,6 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(s, ilx, tl, Wl);
             fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
-            save_f<T, SAVE>(fs, fstride, s, Ft);
             row_phases<T, SET>(ph, g, s + 1);
             chain(std::integral_constant<int, L>{});
         }
@@ -503,7 +489,6 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(1, ilx, tl, Wl);
             fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
-            save_f<T, SAVE>(fs, fstride, 1, Ft);
             row_phases<T, SET>(ph, g, 2);
 #pragma unroll
             for (int a = 0; a < L0; ++a)
@@ -590,9 +575,25 @@ __global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, in
     const int k = blockIdx.x * FM_WAVE + lane;
     double acc = 0.0;
     if (k < nk) {
-        for (int t = seg_ptr[s] + w; t < seg_ptr[s + 1]; t += SEG_WAVES) {
-            const int p = seg_paths[t];
-            acc += deg[p] * (double)chi_path[(size_t)p * nk + k];
+        const int t0 = seg_ptr[s] + w, t1 = seg_ptr[s + 1];
+        constexpr int U = 8;   // paths whose loads are in flight together; summed in list order
+        for (int tb = t0; tb < t1; tb += U * SEG_WAVES) {
+            int pv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = tb + u * SEG_WAVES;
+                pv[u] = t < t1 ? seg_paths[t] : -1;
+            }
+            double dv[U], cv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int p = pv[u] < 0 ? 0 : pv[u];
+                dv[u] = pv[u] < 0 ? 0.0 : deg[p];
+                cv[u] = pv[u] < 0 ? 0.0 : (double)chi_path[(size_t)p * nk + k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (pv[u] >= 0) acc += dv[u] * cv[u];
         }
     }
     part[w][lane] = acc;

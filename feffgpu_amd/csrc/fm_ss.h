@@ -219,21 +219,22 @@ __device__ __forceinline__ V ss_trace(const double *eta, const V (&Tk)[SS_MAXR],
     return acc;
 }
 
-// dL/deta_1 and dL/deta_2 with pbc = conj(dL/dptrac): sum over the pairs of m Im(pbc term) (the values of V)
+// The eta derivatives of tr: d tr / d eta_1 = -i h1, d tr / d eta_2 = -i h2 with h1 (h2) = the sum over the pairs of
+// m_a (m_b) times the pair's term (the values of V).
 template <class T, int ILI, class V>
 __device__ __forceinline__ void ss_trace_eta(const double *eta, const V (&Tk)[SS_MAXR], const V (&Fk)[SS_MAXR],
-                                             cx<T> pbc, T &g1, T &g2) {
+                                             cx<T> &h1, cx<T> &h2) {
     constexpr SSDef D = ssdef(ILI);
     cx<T> p1[7], p2[7];
 #pragma unroll
     for (int m = -3; m <= 3; ++m) { p1[m + 3] = eta_phase<T>(eta[1], m); p2[m + 3] = eta_phase<T>(eta[2], m); }
-    g1 = g2 = T(0);
+    h1 = h2 = mk<T>(T(0), T(0));
 #pragma unroll
     for (int p = 0; p < D.npair; ++p) {
-        const cx<T> term = pbc * (VOps<V>::val(Tk[D.kab[p]]) * VOps<V>::val(Fk[D.kba[p]])) *
+        const cx<T> term = (VOps<V>::val(Tk[D.kab[p]]) * VOps<V>::val(Fk[D.kba[p]])) *
                            (p1[D.M[D.pa[p]] + 3] * p2[D.M[D.pb[p]] + 3]);
-        g1 += T(D.M[D.pa[p]]) * term.i;
-        g2 += T(D.M[D.pb[p]]) * term.i;
+        h1 = h1 + term * T(D.M[D.pa[p]]);
+        h2 = h2 + term * T(D.M[D.pb[p]]);
     }
 }
 

@@ -77,13 +77,11 @@ class PathPlan:
     q: the output grid (1/A). atoms (P, nmax-1) scatterer indices (padding beyond nleg-1 is ignored), nleg (P,),
     absorber (P,), ipots (P, nmax) (absorber first), deg (P,), segment (P,) (default 0), edge (P,) (default 0),
     sigma2 (P,) or a number (default none). cell: None, a cubic length or a 3 x 3 cell. precision 32 or 64.
-    save_intermediates: the forward keeps each path's cchi and scattering matrices per energy on the device
-    (fm_saved_bytes: ~28 KB per 4-leg path at E = 58 in FP32) and the adjoint at the same positions reads them
-    instead of recomputing the amplitude.
+    The adjoint recomputes the amplitudes: nothing is kept between forward and backward.
     """
 
     def __init__(self, phases, q, atoms, nleg, absorber, ipots, deg, segment=None, nseg=None, edge=None, sigma2=None,
-                 cell=None, precision=32, device="cuda", iorder=2, save_intermediates=False):
+                 cell=None, precision=32, device="cuda", iorder=2):
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
             raise RuntimeError("PathPlan runs on a ROCm GPU only")
@@ -198,12 +196,6 @@ class PathPlan:
         self._h_kind = (N._i32 * max(1, len(self.buckets.kind)))(*self.buckets.kind.tolist())
         self._h_edge = (N._i32 * max(1, len(self.buckets.edge)))(*self.buckets.edge.tolist())
         self._h_nlm = (N._i32 * max(1, len(self.buckets.nleg)))(*self.buckets.nleg.tolist())
-        self.saved = None
-        self._saved_for = None      # (data pointer, version) of the positions the saved intermediates belong to
-        self._saved_bucket = None   # (bucket, positions key) after forward_bucket: only that bucket's records
-        if save_intermediates and Pn:
-            nb = N.lib().fm_saved_bytes(self.precision, nmax, max(T.E for T in self.tables), Pn)
-            self.saved = torch.empty(nb, dtype=torch.uint8, device=dev)
 
     def _paths_struct(self, pos64):
         s = N.FmPaths()
@@ -220,8 +212,7 @@ class PathPlan:
 
     def forward(self, pos, want_seg=True):
         """pos (N, 3) A. Returns (chi_seg (nseg, NK) float64, chi_path (P, NK) in the plan's precision, reff (P,) A).
-        chi_path and reff are the plan's own buffers, overwritten by the next call. With save_intermediates, the
-        intermediates for an adjoint at these positions are kept."""
+        chi_path and reff are the plan's own buffers, overwritten by the next call."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         if pos64.dim() != 2 or pos64.shape[1] != 3:
             raise ValueError("pos must be (N, 3)")
@@ -234,23 +225,14 @@ class PathPlan:
                                          self.order_t.data_ptr(), len(self.buckets.kind), self._h_off, self._h_kind,
                                          self._h_edge, self._h_nlm, self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(),
                                          N.ptr(chi_seg), self.chi_path.data_ptr(), self.reff.data_ptr(),
-                                         N.ptr(self.saved), self.saved.numel() if self.saved is not None else 0,
                                          N.stream_of(self.device)), "fm_chi_paths")
-            self._saved_for = self._pos_key(pos64) if self.saved is not None else None
-            self._saved_bucket = None
         elif want_seg:
             chi_seg.zero_()
         return chi_seg, self.chi_path, self.reff
 
-
-    @staticmethod
-    def _pos_key(pos64):
-        return (pos64.data_ptr(), pos64._version, tuple(pos64.shape))
-
     def backward(self, pos, gseg=None, gpath=None):
         """dL/dpos (N, 3) float64 given gseg = dL/dchi_seg (nseg, NK) and/or gpath = dL/dchi_path (P, NK), at the
-        positions pos. The amplitude is read from the intermediates of the last forward when they belong to the same
-        positions tensor, and recomputed inside the kernel otherwise."""
+        positions pos (the amplitudes are recomputed inside the kernel)."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         if self.max_atom >= pos64.shape[0]:
             raise ValueError("pos has too few rows for the path list")
@@ -265,13 +247,11 @@ class PathPlan:
         if gp is not None and tuple(gp.shape) != (self.npath, self.nk):
             raise ValueError("gpath must be (P, NK)")
         ps = self._paths_struct(pos64)
-        use = self.saved is not None and self._saved_for == self._pos_key(pos64)
         N.check(N.lib().fm_chi_paths_backward(self.edge_array, len(self.edges), ps, self.grid, self.precision,
                                               self.iorder, self.order_t.data_ptr(), len(self.buckets.kind), self._h_off,
                                               self._h_kind, self._h_edge, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
                                               N.ptr(gp), self.natom_ref, self.atom_ptr_t.data_ptr(),
                                               self.atom_refs_t.data_ptr(), self.gcoord.data_ptr(), grad.data_ptr(),
-                                              N.ptr(self.saved) if use else None, self.saved.numel() if use else 0,
                                               self.workspace.data_ptr(), self.workspace.numel(),
                                               N.stream_of(self.device)), "fm_chi_paths_backward")
         return grad
@@ -319,17 +299,12 @@ class PathPlan:
         ps = self._paths_struct(pos64)
         N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
                                      self.order_t.data_ptr(), 1, off, kind, edge, nlm, 0, None, None, None,
-                                     self.chi_path.data_ptr(), self.reff.data_ptr(), N.ptr(self.saved),
-                                     self.saved.numel() if self.saved is not None else 0, N.stream_of(self.device)),
+                                     self.chi_path.data_ptr(), self.reff.data_ptr(), N.stream_of(self.device)),
                 "fm_chi_paths")
-        self._saved_for = None
-        self._saved_bucket = (b, self._pos_key(pos64)) if self.saved is not None else None
         return self.chi_path
 
-
     def backward_bucket(self, pos, gseg, b):
-        """The adjoint of bucket b only (gcoord of its paths; no per-atom sum): one kernel launch, for timing. Reads
-        the saved intermediates when the last forward (or forward_bucket) ran at these positions."""
+        """The adjoint of bucket b only (gcoord of its paths; no per-atom sum): one kernel launch, for timing."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         gs = torch.as_tensor(gseg).to(device=self.device, dtype=torch.float64).contiguous()
         off = (N._i32 * 2)(int(self.buckets.offset[b]), int(self.buckets.offset[b + 1]))
@@ -337,14 +312,10 @@ class PathPlan:
         edge = (N._i32 * 1)(int(self.buckets.edge[b]))
         nlm = (N._i32 * 1)(int(self.buckets.nleg[b]))
         ps = self._paths_struct(pos64)
-        key = self._pos_key(pos64)
-        use = self.saved is not None and (self._saved_for == key or self._saved_bucket == (b, key))
         N.check(N.lib().fm_chi_paths_backward(self.edge_array, len(self.edges), ps, self.grid, self.precision,
                                               self.iorder, self.order_t.data_ptr(), 1, off, kind, edge, nlm,
                                               self.segment_t.data_ptr(), gs.data_ptr(), None, 0, None, None,
-                                              self.gcoord.data_ptr(), None,
-                                              N.ptr(self.saved) if use else None, self.saved.numel() if use else 0,
-                                              self.workspace.data_ptr(), self.workspace.numel(),
+                                              self.gcoord.data_ptr(), None, self.workspace.data_ptr(), self.workspace.numel(),
                                               N.stream_of(self.device)), "fm_chi_paths_backward")
         return self.gcoord
 

@@ -131,35 +131,28 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
  * legs (host, 2..nmax; a path whose nleg differs gets NaN for its chi_path row and reff).
  * seg_ptr [nseg + 1] and seg_paths [npath] (device) list the paths of each segment (CSR). chi_path (float for
  * precision 32, double for 64) is required: it is also the workspace of the deterministic segment sum.
- * chi_seg (double) and reff (double, A) may be NULL. saved (may be NULL, saved_bytes >= fm_saved_bytes) receives
- * the intermediates the adjoint can reuse instead of recomputing the amplitude (see fm_saved_bytes). */
+ * chi_seg (double) and reff (double, A) may be NULL. Nothing is kept for the adjoint, which recomputes the
+ * amplitudes. */
 enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_SS = 2 };
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
                  int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,
                  const int32_t *bucket_kind, const int32_t *bucket_edge, const int32_t *bucket_nleg, int32_t nseg,
                  const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff,
-                 void *saved, size_t saved_bytes, void *stream);
-
-/* Bytes of the intermediates a path-list forward saves for its adjoint: per path (record p * (1 + (nmax - 1) * 20) *
- * emax complex elements, float pairs for precision 32, double pairs for 64) cchi[E] and then, for s = 1..nleg-1, the
- * 20 independent entries of the scattering matrix F_s over the energies: F[s-1][k][e]. emax = the largest E of the
- * edges. No counterpart in the reference (its autograd recomputes the forward under checkpointing,
- * src/feffgpu/exafs/forward.py:683). */
-size_t fm_saved_bytes(int32_t precision, int32_t nmax, int32_t emax, int32_t npath);
+                 void *stream);
 
 /* Adjoint of fm_chi_paths: the same path list, buckets and grid (grid.qorder/qstart required); gseg [nseg][nk] =
  * dL/dchi_seg (double) with segment [npath] the row of each path, gpath [npath][nk] = dL/dchi_path (element type by
  * precision) or NULL. Writes gcoord [npath][nmax][3] (double; dL/dcoords of each path's scatterers in slots
  * 1..nleg-1 and minus their sum, the absorber's share, in slot 0) and grad_pos [natom][3] = dL/dpos, the sum over
  * atom_ptr [natom + 1] / atom_refs (CSR of path * nmax + slot references, device), in list order (a path whose
- * nleg differs from its bucket's gets NaN). With saved = the buffer a forward call at the same positions filled,
- * the amplitude is read from it; with saved = NULL it is recomputed inside. */
+ * nleg differs from its bucket's gets NaN). The amplitudes and scattering matrices are recomputed inside (nothing is
+ * kept from the forward call), as the reference's checkpointed autograd does (src/feffgpu/exafs/forward.py:683). */
 int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid,
                           int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
                           const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          const void *saved, size_t saved_bytes, void *workspace, size_t workspace_bytes, void *stream);
+                          void *workspace, size_t workspace_bytes, void *stream);
 
 /* The plain FP64 path stage: the reference's op sequence (src/feffgpu/stage.py:147-388) with one kernel per stage
  * and every intermediate (rotation matrices, spherical-wave factors, gam / gamtl, scattering matrices, termination,

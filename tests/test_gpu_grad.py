@@ -159,26 +159,3 @@ def test_chi_paths_gradient_fd_and_precisions(case, npot):
     assert err32 < 1e-4
     # a periodic box has no net force from a translation of every atom
     assert np.abs(grads[64].sum(0)).max() < 1e-9 * np.abs(grads[64]).sum()
-
-
-@pytest.mark.parametrize("prec,tol", [(64, 1e-11), (32, 2e-5)], ids=["fp64", "fp32"])
-def test_saved_intermediates_adjoint_equals_recompute(prec, tol):
-    """The adjoint reading the forward's saved cchi and scattering matrices equals the one recomputing them, also
-    after a forward at new positions (the saved records follow the positions they were made at)."""
-    from feffgpu_amd.paths import PathPlan
-    phase = GOLDEN / "validation" / "A018" / "phase.pad"
-    q = np.arange(0.0, 18.0001, 0.05)
-    pos, box, pl = _box(10, [1, 20])
-    gseg = torch.tensor(np.random.default_rng(3).normal(size=(2, len(q))) * q ** 2, device="cuda")
-    kw = dict(segment=pl.segment, nseg=2, cell=box[0], precision=prec, device="cuda")
-    ref = PathPlan(str(phase), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, **kw)
-    sav = PathPlan(str(phase), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, save_intermediates=True, **kw)
-    for shift in (0.0, 0.02):
-        p = torch.tensor(pos + shift * np.sin(np.arange(pos.size)).reshape(pos.shape), device="cuda")
-        a = ref.backward(p, gseg).cpu().numpy()
-        sav.forward(p)
-        assert sav._saved_for is not None
-        b = sav.backward(p, gseg).cpu().numpy()
-        err = np.abs(a - b).max() / np.abs(a).max()
-        print(f"[grad] saved vs recomputed adjoint, prec {prec}, shift {shift}: {err:.2e}", flush=True)
-        assert err < tol

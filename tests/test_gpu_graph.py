@@ -1,6 +1,5 @@
 """The step captured as a hipGraph (feffgpu_amd.graph.CapturedStep): every replay equals the eager forward + adjoint
-at the same positions, bit for bit, with eager work on the same plan and host synchronisations between replays; both
-adjoint modes (recomputed amplitudes and the forward's saved intermediates)."""
+at the same positions, bit for bit, with eager work on the same plan and host synchronisations between replays."""
 import numpy as np
 import pytest
 import torch
@@ -13,7 +12,7 @@ PHASE = GOLDEN / "validation" / "A018" / "phase.pad"
 Q = np.arange(0, 18.0001, 0.05)
 
 
-def _setup(save):
+def _setup():
     from feffgpu_amd.paths import PathPlan
     pos0, box = W.lattice("bcc", 2.87, 4)
     ipot = np.full(len(pos0), 2)
@@ -21,14 +20,13 @@ def _setup(save):
     pos = W.displaced(pos0, 0.07, 31)
     pl = W.concat([W.enumerate_paths(pos, a, ipot, 4.8, 4, box=box, segment=i, nmax=4) for i, a in enumerate([0, 9, 77])], 4)
     plan = PathPlan(str(PHASE), Q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, segment=pl.segment, nseg=3,
-                    cell=box[0], precision=32, device="cuda", save_intermediates=save)
+                    cell=box[0], precision=32, device="cuda")
     return pos0, plan
 
 
-@pytest.mark.parametrize("save", [False, True], ids=["recompute", "saved"])
-def test_captured_step_replays_equal_eager(save):
+def test_captured_step_replays_equal_eager():
     from feffgpu_amd.graph import CapturedStep
-    pos0, plan = _setup(save)
+    pos0, plan = _setup()
     q4 = torch.tensor(Q ** 4, device="cuda")
     target = torch.tensor(np.random.default_rng(1).normal(size=(3, len(Q))) * 1e-3, device="cuda")
 
@@ -47,7 +45,7 @@ def test_captured_step_replays_equal_eager(save):
     for rep, i in enumerate((0, 1, 0, 1)):
         seg, g = step(poss[i])
         torch.cuda.synchronize()
-        print(f"[graph] save={save} replay {rep} positions {i}", flush=True)
+        print(f"[graph] replay {rep} positions {i}", flush=True)
         assert torch.equal(seg, ref[i][0]), (rep, i)
         assert torch.equal(g, ref[i][1]), (rep, i)
         eager(poss[1 - i])                       # eager work on the same plan between replays
@@ -56,7 +54,7 @@ def test_captured_step_replays_equal_eager(save):
 
 def test_captured_forward_only_and_static_gseg():
     from feffgpu_amd.graph import CapturedStep
-    pos0, plan = _setup(False)
+    pos0, plan = _setup()
     p = torch.tensor(W.displaced(pos0, 0.07, 50), device="cuda")
     seg_ref = plan.forward(p)[0].clone()
     gs = torch.tensor(np.random.default_rng(2).normal(size=(3, len(Q))), device="cuda")
