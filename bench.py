@@ -126,6 +126,19 @@ def dominant_bucket(plan, pl):
     return int(np.argmax(work)), work
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary (profiles/<kernel>_md_batch.md, a line
+    'traffic_bytes_per_launch: N' written from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy's
+    readings), or None when there is none."""
+    f = ROOT / "profiles" / f"{kernel}_md_batch.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def copy_only(torch, steps):
     """A device-to-device copy of 1 GiB, `steps` times: the known traffic against which the FETCH_SIZE and
     WRITE_SIZE counters are calibrated."""
@@ -264,11 +277,14 @@ def main():
             "(non-packed VALU ceiling 78.6)")
     ach_b = 2 * b_work[bdom] / (t_bwd * 1e-3) / 1e12
     ach_f = b_work[bdom] / (t_fwd * 1e-3) / 1e12
-    roof = dict(bound="mfma", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak, traffic=None,
-                kernel=f"k_backward<float,{int(plan32.buckets.kind[bdom])},*> ({what})", launch_ms=t_bwd,
+    nl = int(plan32.buckets.nleg[bdom])
+    kset = int(plan32.buckets.kind[bdom]) if nl > 2 else 3 + plan32.tables[int(plan32.buckets.edge[bdom])].ilinit - 1
+    roof = dict(bound="mfma", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak,
+                traffic=measured_traffic("k_adjoint") if (kset, nl) == (0, 4) else None,
+                kernel=f"k_adjoint<float,{kset},{nl},*> ({what})", launch_ms=t_bwd,
                 flops_per_launch=2 * b_work[bdom], note=note)
     roof_fwd = dict(achieved=ach_f, peak=peak, unit="TFLOP/s", frac=ach_f / peak,
-                    kernel=f"k_forward<float,{int(plan32.buckets.kind[bdom])}> ({what})", launch_ms=t_fwd,
+                    kernel=f"k_forward<float,{kset}> ({what})", launch_ms=t_fwd,
                     flops_per_launch=b_work[bdom])
 
     out = dict(metric=METRIC, value=units / (ms_fb32 * 1e-3), unit="paths·k-pts/s", n_gpus=1, steps=args.steps,
