@@ -42,7 +42,8 @@ class FmPaths(ctypes.Structure):
 
 
 class FmGrid(ctypes.Structure):
-    _fields_ = [("nk", _i32), ("q", _vp), ("jq", _vp), ("fq", _vp), ("qorder", _vp), ("qstart", _vp), ("qstride", _i32)]
+    _fields_ = [("nk", _i32), ("q", _vp), ("jq", _vp), ("fq", _vp), ("qorder", _vp), ("qstart", _vp), ("qstride", _i32),
+                ("S", _vp), ("sstride", _i32)]
 
 
 def grid_runs(q, jq_rows, E_list):

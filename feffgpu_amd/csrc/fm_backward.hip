@@ -945,7 +945,8 @@ template <class T>
 __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, const double *fq, const double *q, const int32_t *qorder,
                                     const int32_t *qstart, double reff_A, double sigma2, const double *gup_d,
                                     double gup_scale, const T *gup_p, const cx<T> *cc, const T *amp, const double *ang,
-                                    cx<T> *cb, int nk = 0, double *qs = nullptr) {
+                                    cx<T> *cb, int nk = 0, double *qs = nullptr, const double *S = nullptr,
+                                    int sstride = 0) {
     const int lane = lane_id();
     // dL/dA and dL/dPhi at q point i from its interpolation (j, f), q and the upstream gradient gu
     auto eval = [&](int j, double f, double qi, double gu, double &gA, double &gP) {
@@ -966,6 +967,46 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
     };
     auto adj = [&](int i, double &gA, double &gP) { eval(jq[i], fq[i], q[i], gup(i), gA, gP); };
     double rb = 0.0;
+    // dL/dcchi at energy e from dL/dA and dL/dPhi there (through amp = |cchi| and the phase)
+    auto finish = [&](int e, double ab, double pb) {
+        const double cr = (double)cc[e].r, ci = (double)cc[e].i;
+        const double c2 = cr * cr + ci * ci, a = (double)amp[e];
+        cx<T> v = mk<T>(T(0), T(0));
+        if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
+        cb[e] = v;
+    };
+    if (S) {   // dense interpolation (qs required): dL/dA, dL/dPhi per q point, then S^T per energy
+        double *sA = qs, *sP = qs + nk;
+        for (int i = lane; i < nk; i += FM_WAVE) {
+            const double *Si = S + (size_t)i * sstride;
+            double sa = 0.0, sp = 0.0;
+            for (int e = 0; e < E; ++e) {
+                sa += Si[e] * (double)amp[e];
+                sp += Si[e] * ang[e];
+            }
+            double ph = sp + 2.0 * q[i] * reff_A;
+            ph -= FM_TWO_PI * rint(ph / FM_TWO_PI);
+            double gu = gup(i);
+            if (sigma2 != 0.0) gu *= exp(-2.0 * sigma2 * q[i] * q[i]);
+            T sn, cs;
+            if constexpr (sizeof(T) == 4) sincosf((float)ph, &sn, &cs); else sincos(ph, &sn, &cs);
+            const double gP = (double)(T)(gu * sa * (double)cs);
+            sA[i] = (double)(T)(gu * (double)sn);
+            sP[i] = gP;
+            rb += 2.0 * q[i] * gP;
+        }
+        wave_sync();
+        for (int e = lane; e < E; e += FM_WAVE) {
+            double ab = 0.0, pb = 0.0;
+            for (int i = 0; i < nk; ++i) {
+                const double s = S[(size_t)i * sstride + e];
+                ab += s * sA[i];
+                pb += s * sP[i];
+            }
+            finish(e, ab, pb);
+        }
+        return wave_sum(rb);
+    }
     if (qs) {
         double *sA = qs, *sP = qs + nk, *sF = qs + 2 * nk;
         const int t1 = qstart[E];
@@ -990,11 +1031,7 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
                 ab += sF[t] * sA[t];
                 pb += sF[t] * sP[t];
             }
-            const double cr = (double)cc[e].r, ci = (double)cc[e].i;
-            const double c2 = cr * cr + ci * ci, a = (double)amp[e];
-            cx<T> v = mk<T>(T(0), T(0));
-            if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
-            cb[e] = v;
+            finish(e, ab, pb);
         }
         return wave_sum(rb);
     }
@@ -1016,11 +1053,7 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
                 ab += fq[i] * gA;
                 pb += fq[i] * gP;
             }
-        const double cr = (double)cc[e].r, ci = (double)cc[e].i;
-        const double c2 = cr * cr + ci * ci, a = (double)amp[e];
-        cx<T> v = mk<T>(T(0), T(0));
-        if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
-        cb[e] = v;
+        finish(e, ab, pb);
     }
     return wave_sum(rb);
 }
@@ -1101,7 +1134,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                                                        a.G.q, a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride,
                                                        g.reff * FM_BOHR, a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr,
                                                        a.src.P.deg[p], gpr, cc, amp, ang, cb, a.nk,
-                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr);
+                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr,
+                                                       a.G.S ? a.G.S + (size_t)a.edge_id * a.nk * a.G.sstride : nullptr,
+                                                       a.G.sstride);
                 if (lane == 0) rbar[j] = rb;
             }
         __syncthreads();
@@ -1299,7 +1334,9 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
                                                        a.G.q, a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride,
                                                        g.reff * FM_BOHR, a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr,
                                                        a.src.P.deg[p], gpr, cc, amp, ang, cb, a.nk,
-                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr);
+                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr,
+                                                       a.G.S ? a.G.S + (size_t)a.edge_id * a.nk * a.G.sstride : nullptr,
+                                                       a.G.sstride);
                 if (lane == 0) rbar[j] = rb;
             } else {
                 const T *gin = reinterpret_cast<const T *>(a.gcchi) + (size_t)p * E * 2;
@@ -1406,7 +1443,7 @@ __global__ void k_atom_grad(int natom, const int32_t *ptr, const int32_t *refs, 
 template <class T>
 __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, int npath, const T *cchi, const double *reff,
                                                                     fm_grid G, const T *gchi, T *gcchi, double *greff,
-                                                                    size_t wave_bytes) {
+                                                                    size_t wave_bytes, int staged) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / FM_WAVE, lane = lane_id();
     const int wpb = blockDim.x / FM_WAVE;
@@ -1416,12 +1453,14 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, i
     T *amp = reinterpret_cast<T *>(base + align16(sizeof(cx<T>) * E));
     double *ang = reinterpret_cast<double *>(base + align16(sizeof(cx<T>) * E) + align16(sizeof(T) * E));
     cx<T> *cb = reinterpret_cast<cx<T> *>(reinterpret_cast<unsigned char *>(ang) + align16(sizeof(double) * E));
+    double *qs = staged ? reinterpret_cast<double *>(reinterpret_cast<unsigned char *>(cb) + align16(sizeof(cx<T>) * E))
+                        : nullptr;
     for (int p = blockIdx.x * wpb + wid; p < npath; p += gridDim.x * wpb) {
         for (int e = lane; e < E; e += FM_WAVE) cc[e] = mk<T>(cchi[((size_t)p * E + e) * 2], cchi[((size_t)p * E + e) * 2 + 1]);
         wave_sync();
         amp_phase<T>(E, cc, amp, ang);
         const double rb = fine_grid_adjoint<T>(E, G.jq, G.fq, G.q, G.qorder, G.qstart, reff[p], 0.0, nullptr, 0.0,
-                                               gchi + (size_t)p * nk, cc, amp, ang, cb);
+                                               gchi + (size_t)p * nk, cc, amp, ang, cb, nk, qs, G.S, G.sstride);
         wave_sync();
         for (int e = lane; e < E; e += FM_WAVE) {
             gcchi[((size_t)p * E + e) * 2] = cb[e].r;
@@ -1605,16 +1644,21 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     const int E = edge->E;
     const size_t es = precision == 32 ? 4 : 8;
-    const size_t wb = align16(2 * es * E) + align16(es * E) + align16(8 * (size_t)E) + align16(2 * es * E);
-    const int wpb = WPB_MAX;
+    if (grid->S && grid->sstride < E) return set_err(FM_EINVAL, "grid.sstride < E");
+    const size_t qsb = sizeof(double) * 3 * (size_t)grid->nk;
+    const int staged = qsb <= FM_QS_MAX ? 1 : 0;
+    if (grid->S && !staged) return set_err(FM_EUNSUPPORTED, "a dense interpolation matrix needs nk <= 1365 in the adjoint");
+    const size_t wb = align16(2 * es * E) + align16(es * E) + align16(8 * (size_t)E) + align16(2 * es * E) +
+                      (staged ? align16(qsb) : 0);
+    const int wpb = wb * WPB_MAX <= 64 * 1024 ? WPB_MAX : 1;
     const int blocks = (npath + wpb - 1) / wpb;
     hipStream_t st = (hipStream_t)stream;
     if (precision == 32)
         hipLaunchKernelGGL(k_chi_fine_bwd<float>, dim3(blocks), dim3(FM_WAVE * wpb), wb * wpb, st, *edge, npath,
-                           (const float *)cchi, reff, *grid, (const float *)gchi, (float *)gcchi, greff, wb);
+                           (const float *)cchi, reff, *grid, (const float *)gchi, (float *)gcchi, greff, wb, staged);
     else
         hipLaunchKernelGGL(k_chi_fine_bwd<double>, dim3(blocks), dim3(FM_WAVE * wpb), wb * wpb, st, *edge, npath,
-                           (const double *)cchi, reff, *grid, (const double *)gchi, (double *)gcchi, greff, wb);
+                           (const double *)cchi, reff, *grid, (const double *)gchi, (double *)gcchi, greff, wb, staged);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_chi_fine_backward launch");
 }
@@ -1634,6 +1678,9 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     }
     if (iorder != 2) return set_err(FM_EUNSUPPORTED, "only iorder = 2 is implemented on the GPU");
     if (!paths || !grid || !gcoord || !grid->qorder || !grid->qstart) return set_err(FM_EINVAL, "NULL paths, grid or gcoord");
+    if (grid->S && grid->sstride < emax) return set_err(FM_EINVAL, "grid.sstride < E");
+    if (grid->S && sizeof(double) * 3 * (size_t)grid->nk > FM_QS_MAX)
+        return set_err(FM_EUNSUPPORTED, "a dense interpolation matrix needs nk <= 1365 in the adjoint");
     if (!gseg && !gpath) return set_err(FM_EINVAL, "no upstream gradient");
     if (gseg && !segment) return set_err(FM_EINVAL, "gseg needs segment");
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");

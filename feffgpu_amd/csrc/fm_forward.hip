@@ -135,7 +135,8 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
                 if (g.ok)
                     fine_grid<T>(ed, a.G.jq + (size_t)a.edge_id * nk, a.G.fq + (size_t)a.edge_id * nk, a.G.q, nk, reff_A,
                                  a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, reinterpret_cast<T *>(base + off_amp),
-                                 reinterpret_cast<double *>(base + off_ang), out);
+                                 reinterpret_cast<double *>(base + off_ang), out,
+                                 a.G.S ? a.G.S + (size_t)a.edge_id * nk * a.G.sstride : nullptr, a.G.sstride);
                 else
                     for (int i = lane; i < nk; i += FM_WAVE) out[i] = (T)__builtin_nan("");
             }
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine(fm_edge ed, int n
             ang[e] = (double)atan2(ci, cr);
         }
         wave_sync();
-        fine_grid<T>(ed, G.jq, G.fq, G.q, G.nk, reff[p], 0.0, amp, ang, chi + (size_t)p * G.nk);
+        fine_grid<T>(ed, G.jq, G.fq, G.q, G.nk, reff[p], 0.0, amp, ang, chi + (size_t)p * G.nk, G.S, G.sstride);
         wave_sync();
     }
 }
@@ -263,6 +264,7 @@ int fm_chi_fine(const fm_edge *edge, int32_t precision, int32_t npath, const voi
     if (npath < 0) return set_err(FM_EINVAL, "npath < 0");
     if (npath == 0) return FM_OK;
     if (!cchi || !reff || !grid || !chi || !grid->q || !grid->jq || !grid->fq) return set_err(FM_EINVAL, "NULL buffer");
+    if (grid->S && grid->sstride < edge->E) return set_err(FM_EINVAL, "grid.sstride < E");
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     const size_t wb = align16((precision == 32 ? 4 : 8) * (size_t)edge->E) + align16(8 * (size_t)edge->E);
     const int wpb = WPB_MAX;
@@ -293,6 +295,8 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
     if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
     if (paths->nmax < 2 || paths->nmax > FM_MAX_LEGS) return set_err(FM_EINVAL, "nmax out of range 2..9");
     if (grid->nk < 1) return set_err(FM_EINVAL, "empty grid");
+    for (int i = 0; i < nedge; ++i)
+        if (grid->S && grid->sstride < edges[i].E) return set_err(FM_EINVAL, "grid.sstride < E");
     if (nbucket > 0 && (!bucket_offset || !bucket_set || !bucket_nleg)) return set_err(FM_EINVAL, "NULL bucket arrays");
     hipStream_t st = (hipStream_t)stream;
     for (int b = 0; b < nbucket; ++b) {

@@ -519,7 +519,7 @@ __device__ cx<T> ss_cchi(const fm_edge &ed, const PathLDS &g, int ee) {
 // amp[] and ang[]), unwrap (FP64 scan), interpolation to q, chi = A sin(Phi + 2 q reff) exp(-2 sigma2 q^2).
 template <class T>
 __device__ void fine_grid(const fm_edge &ed, const int32_t *jq, const double *fq, const double *q, int nk, double reff_A,
-                          double sigma2, T *amp, double *ang, T *out) {
+                          double sigma2, T *amp, double *ang, T *out, const double *S = nullptr, int sstride = 0) {
     const int lane = lane_id();
     const int E = ed.E;
     // unwrap: pha[e] = ang[0] + sum_{i<=e} wrap(ang[i] - ang[i-1]) (src/feffgpu/stage.py:365-370)
@@ -547,10 +547,23 @@ __device__ void fine_grid(const fm_edge &ed, const int32_t *jq, const double *fq
     }
     wave_sync();
     for (int i = lane; i < nk; i += FM_WAVE) {
-        const int j = jq[i];
-        const double f = fq[i];
-        const T A = (T)((1.0 - f) * (double)amp[j] + f * (double)amp[j + 1]);
-        double ph = (1.0 - f) * ang[j] + f * ang[j + 1] + 2.0 * q[i] * reff_A;
+        T A;
+        double ph;
+        if (S) {   // dense interpolation: row i of S (this edge's block)
+            const double *Si = S + (size_t)i * sstride;
+            double sa = 0.0, sp = 0.0;
+            for (int e = 0; e < E; ++e) {
+                sa += Si[e] * (double)amp[e];
+                sp += Si[e] * ang[e];
+            }
+            A = (T)sa;
+            ph = sp + 2.0 * q[i] * reff_A;
+        } else {
+            const int j = jq[i];
+            const double f = fq[i];
+            A = (T)((1.0 - f) * (double)amp[j] + f * (double)amp[j + 1]);
+            ph = (1.0 - f) * ang[j] + f * ang[j + 1] + 2.0 * q[i] * reff_A;
+        }
         ph -= FM_TWO_PI * rint(ph / FM_TWO_PI);
         T v;
         if constexpr (sizeof(T) == 4) v = A * sinf((float)ph);

@@ -364,7 +364,8 @@ __global__ __launch_bounds__(FM_WAVE * 4) void k_pl_fine(PDims d, fm_edge ed, fm
     }
     wave_sync();
     fine_grid<double>(ed, G.jq + (size_t)edge_id * nk, G.fq + (size_t)edge_id * nk, G.q, nk, reff_A,
-                      sigma2 ? sigma2[p] : 0.0, amp, ang, out);
+                      sigma2 ? sigma2[p] : 0.0, amp, ang, out, G.S ? G.S + (size_t)edge_id * nk * G.sstride : nullptr,
+                      G.sstride);
 }
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }

@@ -22,7 +22,7 @@ import torch
 
 from . import _native as N
 from .pad import load_phase
-from .tables import edge_tables, energies_needed, interpolation
+from .tables import edge_tables, energies_needed, interpolation, spline_matrix
 
 
 def _cell_struct(cell):
@@ -77,17 +77,22 @@ class PathPlan:
     q: the output grid (1/A). atoms (P, nmax-1) scatterer indices (padding beyond nleg-1 is ignored), nleg (P,),
     absorber (P,), ipots (P, nmax) (absorber first), deg (P,), segment (P,) (default 0), edge (P,) (default 0),
     sigma2 (P,) or a number (default none). cell: None, a cubic length or a 3 x 3 cell. precision 32 or 64.
+    convention: "feff" (linear interpolation of amplitude and phase, FEFF's, as the reference's forward model uses)
+    or "larch" (the cubic spline_matrix of every edge over all its phase-grid energies, a dense interpolation).
     The adjoint recomputes the amplitudes: nothing is kept between forward and backward.
     """
 
     def __init__(self, phases, q, atoms, nleg, absorber, ipots, deg, segment=None, nseg=None, edge=None, sigma2=None,
-                 cell=None, precision=32, device="cuda", iorder=2):
+                 cell=None, precision=32, device="cuda", iorder=2, convention="feff"):
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
             raise RuntimeError("PathPlan runs on a ROCm GPU only")
         if precision not in (32, 64):
             raise ValueError("precision must be 32 or 64")
         self.precision, self.iorder = int(precision), int(iorder)
+        if convention not in ("feff", "larch"):
+            raise ValueError("convention must be 'feff' or 'larch'")
+        self.convention = convention
         self.rdtype = torch.float32 if precision == 32 else torch.float64
         phases = phases if isinstance(phases, (list, tuple)) else [phases]
         self.phases = [load_phase(p) for p in phases]
@@ -126,7 +131,8 @@ class PathPlan:
         self.tables, self.edges = [], []
         jq, fq = [], []
         for ph in self.phases:
-            E = energies_needed(ph, self.q)
+            # the linear interpolation reads the energies up to the grid's last upper point; a spline all of them
+            E = energies_needed(ph, self.q) if convention == "feff" else ph.ne1
             T = edge_tables(ph, E)
             self.tables.append(T)
             self.edges.append(N.DeviceEdge(T, self.device))
@@ -144,6 +150,14 @@ class PathPlan:
         self.grid = N.FmGrid()
         self.grid.nk, self.grid.q, self.grid.jq, self.grid.fq = nk, self.q_t.data_ptr(), self.jq_t.data_ptr(), self.fq_t.data_ptr()
         self.grid.qorder, self.grid.qstart, self.grid.qstride = self.qorder_t.data_ptr(), self.qstart_t.data_ptr(), qstride
+        self.S_t = None
+        if convention == "larch":
+            emax = max(T.E for T in self.tables)
+            S = np.zeros((len(self.tables), nk, emax))
+            for e, T in enumerate(self.tables):
+                S[e, :, :T.E] = spline_matrix(T.kf, self.q)
+            self.S_t = torch.tensor(S, device=dev)
+            self.grid.S, self.grid.sstride = self.S_t.data_ptr(), emax
         # path arrays
         self.npath, self.nmax, self.nk = Pn, nmax, nk
         self.nleg_h, self.edge_h, self.segment_h, self.deg_h = nleg, edge, segment, deg

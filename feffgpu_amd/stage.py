@@ -21,6 +21,7 @@ import torch
 from . import _native as N
 from .pad import BOHR, PhaseData, load_phase, read_phase_pad
 from .tables import edge_tables, interpolation, setkap
+from .tables import spline_matrix as tables_spline_matrix
 
 
 def _require_gpu(device: torch.device):
@@ -84,6 +85,7 @@ class GPUPathStage:
         self.tables = edge_tables(P, E)
         self._edge = N.DeviceEdge(self.tables, self.device)
         self._grids = {}
+        self._linear, self._spline = {}, {}
         N.lib()
 
     # -- the path signal on the phase grid ----------------------------------------------------------------
@@ -130,24 +132,30 @@ class GPUPathStage:
     def spline_matrix(self, q):
         """Cubic-spline interpolation matrix (len(q), E) (scipy UnivariateSpline s=0, as Larch); q clamped to the
         grid. Host-built once, as in src/feffgpu/stage.py:341-351."""
-        from scipy.interpolate import UnivariateSpline
-        kf = self.P.xk[: self.E] / BOHR
-        qn = np.clip(np.asarray(q, float), kf[0], kf[-1])
-        S = np.zeros((len(qn), self.E))
-        for j in range(self.E):
-            y = np.zeros(self.E)
-            y[j] = 1.0
-            S[:, j] = UnivariateSpline(kf, y, s=0, k=3)(qn)
+        S = tables_spline_matrix(self.P.xk[: self.E] / BOHR, q)
         return torch.tensor(S, device=self.device, dtype=self.rdtype)
 
     def linear_matrix(self, q):
-        """Linear interpolation matrix (len(q), E) of FEFF's final-stage convention; q clamped to the grid."""
-        kf = self.P.xk[: self.E] / BOHR
-        j, f = interpolation(kf, q)
-        S = np.zeros((len(j), self.E))
-        S[np.arange(len(j)), j] = 1 - f
-        S[np.arange(len(j)), j + 1] = f
-        return torch.tensor(S, device=self.device, dtype=self.rdtype)
+        """Linear interpolation matrix (len(q), E) of FEFF's final-stage convention; q clamped to the grid. The
+        returned tensor is cached per grid: chi_fine recognises it and runs the kernels' own linear interpolation."""
+        key = self._qkey(q)
+        S = self._linear.get(key)
+        if S is None:
+            kf = self.P.xk[: self.E] / BOHR
+            j, f = interpolation(kf, q)
+            m = np.zeros((len(j), self.E))
+            m[np.arange(len(j)), j] = 1 - f
+            m[np.arange(len(j)), j + 1] = f
+            S = self._linear[key] = torch.tensor(m, device=self.device, dtype=self.rdtype)
+        return S
+
+    @staticmethod
+    def _qarray(q):
+        return np.ascontiguousarray(np.asarray(q.detach().cpu() if torch.is_tensor(q) else q, dtype=np.float64))
+
+    def _qkey(self, q):
+        qa = self._qarray(q)
+        return qa.shape[0], hash(qa.tobytes())
 
     @staticmethod
     def unwrap(phase):
@@ -156,7 +164,9 @@ class GPUPathStage:
         d = d - 2 * math.pi * torch.round(d / (2 * math.pi))
         return torch.cat([phase[..., :1], phase[..., :1] + torch.cumsum(d, -1)], -1)
 
-    def _grid(self, q):
+    def _grid(self, q, S=None):
+        """The kernels' grid for q: linear interpolation (S None; cached per grid) or the dense matrix S (len(q), E),
+        a grid of its own per call that holds S in FP64 for as long as the autograd graph keeps it."""
         qa = np.ascontiguousarray(np.asarray(q.detach().cpu() if torch.is_tensor(q) else q, dtype=np.float64))
         key = (qa.shape[0], hash(qa.tobytes()))
         g = self._grids.get(key)
@@ -173,16 +183,34 @@ class GPUPathStage:
             s.nk, s.q, s.jq, s.fq = len(qa), qt.data_ptr(), jt.data_ptr(), ft.data_ptr()
             s.qorder, s.qstart, s.qstride = ot.data_ptr(), st.data_ptr(), self.E + 1
             g = self._grids[key] = (s, qt, jt, ft, ot, st)
-        return g
+        if S is None:
+            return g
+        dense = torch.as_tensor(S).detach().to(device=self.device, dtype=torch.float64).contiguous()
+        if tuple(dense.shape) != (len(qa), self.E):
+            raise ValueError(f"S must be (len(q), E) = ({len(qa)}, {self.E})")
+        s = N.FmGrid.from_buffer_copy(g[0])
+        s.S, s.sstride = dense.data_ptr(), self.E
+        return (s,) + g[1:] + (dense,)
 
     def chi_fine(self, cchi, reff, q, S=None, convention="feff"):
-        """chi(q) = A(q) sin(Phi(q) + 2 q reff) per path, A and Phi the linear interpolations of |cchi| and of its
-        unwrapped phase (FEFF's convention). Returns (B, len(q)) in the stage's real dtype; differentiable in cchi
-        and reff. ``S`` (the matching linear_matrix) is accepted for compatibility; the kernel interpolates itself."""
-        if convention != "feff":
-            raise NotImplementedError("only the 'feff' (linear) convention runs on the GPU")
-        grid = self._grid(q)
-        return _ChiFine.apply(cchi, torch.as_tensor(reff, device=self.device), self, grid)
+        """chi(q) = A(q) sin(Phi(q) + 2 q reff) per path, A and Phi |cchi| and its unwrapped phase interpolated from
+        the phase grid by S (src/feffgpu/stage.py:372-388): linear_matrix(q) for convention "feff" (FEFF's), the
+        cubic spline_matrix(q) for "larch", or the S given. Returns (B, len(q)) in the stage's real dtype;
+        differentiable in cchi and reff. The matrix linear_matrix returned runs as the kernels' own linear
+        interpolation; any other S as a dense interpolation matrix."""
+        if S is None:
+            if convention == "feff":
+                return _ChiFine.apply(cchi, torch.as_tensor(reff, device=self.device), self, self._grid(q))
+            if convention != "larch":
+                raise ValueError("convention must be 'feff' or 'larch'")
+            key = self._qkey(q)
+            S = self._spline.get(key)
+            if S is None:     # FP64 whatever the stage's dtype: the kernels interpolate in FP64
+                kf = self.P.xk[: self.E] / BOHR
+                S = self._spline[key] = torch.tensor(tables_spline_matrix(kf, self._qarray(q)), device=self.device)
+        elif S is self._linear.get(self._qkey(q)):
+            S = None
+        return _ChiFine.apply(cchi, torch.as_tensor(reff, device=self.device), self, self._grid(q, S))
 
     def _chi_fine_raw(self, cchi, reff, grid):
         c = cchi.detach().to(self.cdtype).contiguous()

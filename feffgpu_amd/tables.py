@@ -122,6 +122,21 @@ def rotation_coefficients(L1: int):
     return rotc, rotn
 
 
+def spline_matrix(kf: np.ndarray, q) -> np.ndarray:
+    """Cubic-spline interpolation matrix (len(q), len(kf)) from the phase grid kf (1/A) to q, q clamped to the grid:
+    column j is scipy's interpolating UnivariateSpline (s = 0, k = 3, as Larch) of the j-th unit vector
+    (src/feffgpu/stage.py:341-351)."""
+    from scipy.interpolate import UnivariateSpline
+    kf = np.asarray(kf, float)
+    qn = np.clip(np.asarray(q, float), kf[0], kf[-1])
+    S = np.zeros((len(qn), len(kf)))
+    for j in range(len(kf)):
+        y = np.zeros(len(kf))
+        y[j] = 1.0
+        S[:, j] = UnivariateSpline(kf, y, s=0, k=3)(qn)
+    return S
+
+
 def interpolation(kf_full: np.ndarray, q: np.ndarray):
     """Lower index j (clipped to 0..E-2) and upper weight f of the linear interpolation of q on the ascending grid
     kf_full (linear_matrix's rule, src/feffgpu/stage.py:356-362: q clamped to the grid, searchsorted right - 1)."""

@@ -91,6 +91,12 @@ typedef struct fm_grid {
     const int32_t *qorder;       /* [nk] indices of q in ascending order */
     const int32_t *qstart;       /* [nedge][qstride] first position in qorder whose jq >= e, e = 0..E_edge */
     int32_t qstride;             /* row length of qstart (>= largest E + 1) */
+    /* optional dense interpolation matrix (GPUPathStage.chi_fine's S, e.g. spline_matrix for the "larch"
+     * convention, src/feffgpu/stage.py:341-351, 372-388): amplitude and unwrapped phase at q[i] of edge ed are
+     * sum_e S[(ed * nk + i) * sstride + e] times their values at energy e; NULL: the linear jq / fq interpolation.
+     * The adjoints need nk <= 1365 with S set. */
+    const double *S;
+    int32_t sstride;             /* row length of S (>= largest E) */
 } fm_grid;
 
 /* GPUPathStage.cchi (src/feffgpu/stage.py:235-338): npath paths of nleg legs each; coords [npath][nleg][3] (A,

@@ -65,6 +65,9 @@ def _check(rc):
         raise RuntimeError("oracle: " + lib().orc_last_error().decode())
 
 
+BOHR = 0.52917721067   # A, as FEFF and the reference (src/feffgpu/pad.py:13)
+
+
 class OraclePhase:
     """A phase.pad file loaded by the oracle's own parser."""
 
@@ -89,6 +92,36 @@ class OraclePhase:
         ck = np.zeros(2 * self.ne1)
         lib().orc_tables(self.h, _p(lm, _ip), _p(xk, _dp), _p(ck, _dp))
         return lm, xk, ck[0::2] + 1j * ck[1::2]
+
+    def spline_matrix(self, q):
+        """Cubic-spline interpolation matrix (len(q), ne1) from the phase grid (1/A) to q, q clamped to the grid:
+        scipy's interpolating UnivariateSpline (s = 0, k = 3) of each unit vector, as Larch and the reference's
+        spline_matrix build it (src/feffgpu/stage.py:341-351)."""
+        from scipy.interpolate import UnivariateSpline
+        kf = self.tables()[1] / BOHR
+        qn = np.clip(np.asarray(q, float), kf[0], kf[-1])
+        S = np.zeros((len(qn), self.ne1))
+        for j in range(self.ne1):
+            y = np.zeros(self.ne1)
+            y[j] = 1.0
+            S[:, j] = UnivariateSpline(kf, y, s=0, k=3)(qn)
+        return S
+
+    @staticmethod
+    def chi_fine_dense(cchi, reff, q, S):
+        """chi on the points q with a dense interpolation matrix S (len(q), E) of amplitude and unwrapped phase
+        (src/feffgpu/stage.py:372-388): amp = sqrt(|cchi|^2 + 1e-30), the phase unwrapped with its successive
+        differences brought to (-pi, pi] by rounding half to even, chi = (amp S^T) sin(phase S^T + 2 q reff).
+        cchi (B, E) or (E,), reff (B,) or a number; returns (B, len(q))."""
+        c = np.atleast_2d(np.asarray(cchi))
+        r = np.broadcast_to(np.asarray(reff, float), (c.shape[0],))
+        amp = np.sqrt(c.real ** 2 + c.imag ** 2 + 1e-30)
+        ang = np.arctan2(c.imag, c.real)
+        d = np.diff(ang, axis=-1)
+        d = d - 2 * np.pi * np.round(d / (2 * np.pi))
+        pha = np.concatenate([ang[:, :1], ang[:, :1] + np.cumsum(d, -1)], -1)
+        S = np.asarray(S, float)
+        return (amp @ S.T) * np.sin(pha @ S.T + 2 * np.asarray(q, float)[None, :] * r[:, None])
 
     def cchi(self, coords, ipots, iorder=2, rotation="exact"):
         """One path: coords (nleg, 3) A, absorber first at the origin; ipots (nleg,). Returns (cchi (ne1,) complex,

@@ -179,3 +179,37 @@ def test_chi_paths_cell_and_segments():
     shifted = pos + L * np.round(np.random.default_rng(2).uniform(-2, 2, pos.shape))
     s2, _, _ = oracle.chi_paths(ph, shifted, atoms, nleg, np.zeros(len(sel)), ipots, deg, k, segment=seg, nseg=3, cell=L)
     np.testing.assert_allclose(s2, s1, rtol=0, atol=1e-9 * np.abs(s1).max())
+
+
+def test_dense_fine_grid_matches_linear_and_spline_interpolates():
+    """The oracle's dense-matrix fine grid (chi_fine_dense, any interpolation matrix S) with FEFF's linear matrix
+    (src/feffgpu/stage.py:353-363) equals the oracle's own C fine grid; its spline matrix has rows summing to 1 and
+    reproduces cubic polynomials exactly (an interpolating cubic spline does)."""
+    phase, paths, _, _ = load_case("validation/A014")
+    orc = oracle.OraclePhase(phase)
+    q = np.arange(0.0, 18.0001, 0.05)
+    kf = orc.tables()[1] / oracle.BOHR
+    qn = np.clip(q, kf[0], kf[-1])
+    j = np.clip(np.searchsorted(kf, qn, side="right") - 1, 0, orc.ne1 - 2)
+    f = (qn - kf[j]) / (kf[j + 1] - kf[j])
+    S = np.zeros((len(q), orc.ne1))
+    S[np.arange(len(q)), j] = 1 - f
+    S[np.arange(len(q)), j + 1] = f
+    for p in paths[:20]:
+        c, r, _ = orc.cchi(p.coords, p.ipots)
+        a = orc.chi_fine(c, r, q)[0]
+        b = orc.chi_fine_dense(c, r, q, S)[0]
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    Ssp = orc.spline_matrix(q)
+    assert np.abs(Ssp.sum(1) - 1.0).max() < 1e-12
+    for deg in (1, 2, 3):
+        assert np.abs(Ssp @ (kf / 10) ** deg - (qn / 10) ** deg).max() < 1e-10
+    # linearity: the matrix applied to random data is scipy's spline of that data (a transposed or shifted column fails)
+    from scipy.interpolate import UnivariateSpline
+    y = np.random.default_rng(4).normal(size=orc.ne1)
+    assert np.abs(Ssp @ y - UnivariateSpline(kf, y, s=0, k=3)(qn)).max() < 1e-9 * np.abs(y).max()
+    # the product's host-built matrices (feffgpu_amd.tables) are these
+    from feffgpu_amd.tables import interpolation, spline_matrix
+    assert np.abs(spline_matrix(kf, q) - Ssp).max() == 0.0
+    jj, ff = interpolation(kf, q)
+    assert np.array_equal(jj, j) and np.abs(ff - f).max() < 1e-15
