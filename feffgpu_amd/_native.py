@@ -121,11 +121,16 @@ def ptr(t) -> int | None:
     return None if t is None else t.data_ptr()
 
 
-def backward_workspace(precision: int, nlegmax: int, npath: int, device, max_slots: int = 4096):
-    """A device buffer for the adjoints' per-lane state: band slots for the workgroups in flight (a workgroup takes
-    one per band of its plan, at most 16), at most max_slots and no more than npath workgroups need."""
+def backward_workspace(precision: int, nlegmax: int, npath: int, device, E: int, max_slots: int = 4096,
+                       max_region_bytes: int = 1 << 30):
+    """A device buffer for the adjoints: band slots for the workgroups in flight (paths of 7-9 legs: a workgroup
+    takes one per band of its plan, at most 16), at most max_slots and no more than npath workgroups need; and, for
+    the fused adjoint (2-6 legs), a region of (4 nleg + 1) E complex per path of a launch plus one spare, for up to
+    npath paths and at most max_region_bytes (larger buckets run in chunks)."""
     slots = max(16, min(int(max_slots), 16 * max(int(npath), 1)))
     nbytes = lib().fm_backward_workspace_bytes(int(precision), int(nlegmax), slots)
+    per = (4 * min(int(nlegmax), 6) + 1) * int(E) * (8 if precision == 32 else 16)
+    nbytes = max(nbytes, min((int(npath) + 1) * per, int(max_region_bytes)))
     return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
 
 

@@ -60,6 +60,8 @@ struct BArgs {
     size_t off_gs;              // 3 n^2 doubles of scratch for the geometry adjoint
     unsigned char *work;        // per band slot: prefix products and scattering matrices, gradient slots
     size_t work_per_wave, work_off_F, work_off_acc, work_bytes;
+    size_t dstride;             // fused adjoint: complex elements of a path's workspace region, (4 nleg + 1) E
+    size_t dbytes;              // fused adjoint: bytes of the launch's regions (its paths and one spare)
 };
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }
@@ -176,6 +178,23 @@ constexpr int FM_RSRC_WORD3 = 0x00020000;   // buffer descriptor word 3 on gfx9:
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(unsigned char *p, size_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, FM_RSRC_WORD3);
 }
+template <class E>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t rs, uint32_t vo, int so, const E &v) {
+    if constexpr (sizeof(E) == 4) {
+        unsigned x;
+        __builtin_memcpy(&x, &v, 4);
+        __builtin_amdgcn_raw_buffer_store_b32(x, rs, vo, so, 0);
+    } else if constexpr (sizeof(E) == 8) {
+        __attribute__((ext_vector_type(2))) unsigned x;
+        __builtin_memcpy(&x, &v, 8);
+        __builtin_amdgcn_raw_buffer_store_b64(x, rs, vo, so, 0);
+    } else {
+        static_assert(sizeof(E) == 16, "element of 4, 8 or 16 bytes");
+        __attribute__((ext_vector_type(4))) unsigned x;
+        __builtin_memcpy(&x, &v, 16);
+        __builtin_amdgcn_raw_buffer_store_b128(x, rs, vo, so, 0);
+    }
+}
 template <class E> struct LaneArray {
     __amdgpu_buffer_rsrc_t rs;
     uint32_t lo;           // lane * sizeof(E)
@@ -195,22 +214,7 @@ template <class E> struct LaneArray {
         }
         return v;
     }
-    __device__ __forceinline__ void st(int elem, const E &v) const {
-        const int so = elem * (int)(FM_WAVE * sizeof(E));
-        if constexpr (sizeof(E) == 4) {
-            unsigned x;
-            __builtin_memcpy(&x, &v, 4);
-            __builtin_amdgcn_raw_buffer_store_b32(x, rs, lo, so, 0);
-        } else if constexpr (sizeof(E) == 8) {
-            __attribute__((ext_vector_type(2))) unsigned x;
-            __builtin_memcpy(&x, &v, 8);
-            __builtin_amdgcn_raw_buffer_store_b64(x, rs, lo, so, 0);
-        } else {
-            __attribute__((ext_vector_type(4))) unsigned x;
-            __builtin_memcpy(&x, &v, 16);
-            __builtin_amdgcn_raw_buffer_store_b128(x, rs, lo, so, 0);
-        }
-    }
+    __device__ __forceinline__ void st(int elem, const E &v) const { buf_st(rs, lo, elem * (int)(FM_WAVE * sizeof(E)), v); }
 };
 
 // Per-lane gradient slots (kind, leg j), each written once per path.
@@ -617,12 +621,14 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
     }
 }
 
-// Complex derivative slots of a lane: (kind, leg) as GSlots.
+// Complex derivative slots of a lane, (kind, leg j) -> slot kind nleg + j - 1, in the workspace region of the lane's
+// path: slot s of the lane's energy at byte lo + s sbytes (lo: the lane's (path, energy) offset in slot 0, sbytes: E
+// complex).
 template <class T> struct DSlots {
-    LaneArray<cx<T>> v;
-    int nleg;
-    __device__ __forceinline__ void put(int kind, int j, cx<T> x) const { v.st(kind * nleg + j - 1, x); }
-    __device__ __forceinline__ cx<T> get(int slot) const { return v.ld(slot); }
+    __amdgpu_buffer_rsrc_t rs;
+    uint32_t lo;
+    int sbytes, nleg;
+    __device__ __forceinline__ void put(int kind, int j, cx<T> x) const { buf_st(rs, lo, (kind * nleg + j - 1) * sbytes, x); }
 };
 
 // F(i, j) of a block from its representatives, with the row phase: ph[i] sg Ft[idx[i][j]]
@@ -1264,168 +1270,168 @@ __device__ __forceinline__ cx<T> ss_fused_energy(const fm_edge &ed, const PathLD
 // leaves cchi in LDS and the complex derivatives of ptrac in the band's workspace; then per path the fine-grid
 // adjoint (or the given dL/dcchi), pbc and the cfac terms per energy; then the derivative slots contracted with pbc
 // and summed over the energies in band order, and the geometry adjoint.
+// The fused adjoint's sweep (one wavefront per SIMD): per block of G paths, the geometry and the rotation tables
+// with their beta derivatives into LDS, then every energy of every path in one sweep. Each path leaves cchi and the
+// 4 nleg complex derivative slots of every energy in its workspace region (path-major: cchi [E], slot s at
+// [(1 + s) E]); k_adjoint_tail takes it from there. Lanes past their band's end and paths past the launch's end
+// compute on a duplicate and write their slots to the spare region after the last path.
 template <class T, int SET, int NL, int WPE>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adjoint(BArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int pidx[8], pval[8];
-    __shared__ double rbar[8];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
     const int lane = lane_id();
     const fm_edge &ed = a.ed;
     const int G = a.gp, cap = FM_WAVE / G;
     const int ltab = ed.L1 + 1, E = ed.E;
     const int nband = a.bp.nband;
-    const uint32_t loc = (uint32_t)lane * sizeof(cx<T>);
     const int jl = lane / cap, il = lane % cap;
     unsigned char *pbase = smem + (size_t)jl * a.path_bytes;
     const PathLDS &gj = *reinterpret_cast<const PathLDS *>(pbase);
     const T *Wj = reinterpret_cast<const T *>(pbase + a.off_W);
     const T *Wdj = reinterpret_cast<const T *>(pbase + a.off_Wd);
-    cx<T> *ccj = reinterpret_cast<cx<T> *>(pbase + a.off_cc);
-    unsigned char *const wg_work = a.work + (size_t)blockIdx.x * nband * a.work_per_wave;
-    const size_t wpw = a.work_per_wave;
+    const __amdgpu_buffer_rsrc_t drs = slice_rsrc(a.work, a.dbytes);
+    cx<T> *const wcc = reinterpret_cast<cx<T> *>(a.work);
+    const int sbytes = E * (int)sizeof(cx<T>);
 
     for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
         // ---- 1. setup: one wavefront per path
         for (int j = wid; j < G; j += 4) {
             const int slot = blk * G + j;
-            const bool valid = slot < a.npath_bucket;
-            const int s0 = valid ? slot : blk * G;
+            const int s0 = slot < a.npath_bucket ? slot : blk * G;
             const int p = a.order ? a.order[a.first + s0] : a.first + s0;
-            unsigned char *base = smem + (size_t)j * a.path_bytes;
-            PathLDS &g = *reinterpret_cast<PathLDS *>(base);
+            PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)j * a.path_bytes);
             int ei;
             load_path(a.src, p, g, ei, NL);
-            if (lane == 0) {
-                pidx[j] = p;
-                pval[j] = valid ? 1 : 0;
-                rbar[j] = (a.src.mode == 0 && a.greff) ? a.greff[p] : 0.0;
-            }
         }
         if constexpr (SET < FM_SET_SS) build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
         __syncthreads();
         // ---- 2. the fused sweep over this wavefront's bands
+        const int sj = blk * G + jl;
+        const bool vj = sj < a.npath_bucket;
         for (int b = 0; b < nband; ++b) {
             if (a.bp.wave[b] != wid) continue;
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
+            const bool on = vj && il < nb;
             const int e = e0 + (il < nb ? il : nb - 1);
-            const DSlots<T> ds{{slice_rsrc(wg_work + (size_t)b * wpw, wpw), loc}, NL};
+            const size_t reg = on ? (size_t)sj : (size_t)a.npath_bucket;
+            const DSlots<T> ds{drs, (uint32_t)((reg * a.dstride + E + e) * sizeof(cx<T>)), sbytes, NL};
             cx<T> c;
             if constexpr (SET >= FM_SET_SS) c = ss_fused_energy<T, SET - FM_SET_SS + 1>(ed, gj, e, ds);
             else c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds);
-            if (il < nb) ccj[e] = c;
+            if (on) wcc[(size_t)sj * a.dstride + e] = c;
         }
         __syncthreads();
-        // ---- 3. per path: dL/dcchi per energy, then pbc = conj(dL/dptrac) (in cb) and the two cfac terms (in amp, ang)
-        for (int j = wid; j < G; j += 4) {
-            if (!pval[j]) continue;
-            unsigned char *base = smem + (size_t)j * a.path_bytes;
-            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
-            const cx<T> *cc = reinterpret_cast<const cx<T> *>(base + a.off_cc);
-            T *amp = reinterpret_cast<T *>(base + a.off_amp);
-            double *ang = reinterpret_cast<double *>(base + a.off_ang);
-            cx<T> *cb = reinterpret_cast<cx<T> *>(base + a.off_cb);
-            const int p = pidx[j];
-            if (a.src.mode == 1) {
-                amp_phase<T>(E, cc, amp, ang);
-                const double *gsr = a.gseg ? a.gseg + (size_t)a.segment[p] * a.nk : nullptr;
-                const T *gpr = a.gpath ? reinterpret_cast<const T *>(a.gpath) + (size_t)p * a.nk : nullptr;
-                const double rb = fine_grid_adjoint<T>(E, a.G.jq + (size_t)a.edge_id * a.nk, a.G.fq + (size_t)a.edge_id * a.nk,
-                                                       a.G.q, a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride,
-                                                       g.reff * FM_BOHR, a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr,
-                                                       a.src.P.deg[p], gpr, cc, amp, ang, cb, a.nk,
-                                                       a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr,
-                                                       a.G.S ? a.G.S + (size_t)a.edge_id * a.nk * a.G.sstride : nullptr,
-                                                       a.G.sstride);
-                if (lane == 0) rbar[j] = rb;
-            } else {
-                const T *gin = reinterpret_cast<const T *>(a.gcchi) + (size_t)p * E * 2;
-                for (int e = lane; e < E; e += FM_WAVE) cb[e] = mk<T>(gin[2 * e], gin[2 * e + 1]);
-            }
-            wave_sync();
-            for (int e = lane; e < E; e += FM_WAVE) {
-                const cx<T> cbar = cb[e];
-                const cx<T> w = conj(cbar) * cc[e];
-                const cx<T> wdk = w * mk<T>((T)ed.ckmxk[2 * e], (T)ed.ckmxk[2 * e + 1]);
-                amp[e] = -wdk.i;                                      // d cfac / d r_j: a1 - a2 / r_j
-                ang[e] = (double)w.r;
-                cb[e] = conj(cbar * conj(cfac_at<T>(ed, g, e, NL)));
-            }
-            wave_sync();
-        }
-        __syncthreads();
-        // ---- 4. per path: Re(pbc D) summed over the energies in band order, the geometry adjoint, the output
-        for (int j = wid; j < G; j += 4) {
-            if (!pval[j]) continue;
-            unsigned char *base = smem + (size_t)j * a.path_bytes;
-            const PathLDS &g = *reinterpret_cast<const PathLDS *>(base);
-            const T *amp = reinterpret_cast<const T *>(base + a.off_amp);
-            const double *ang = reinterpret_cast<const double *>(base + a.off_ang);
-            const cx<T> *pbc = reinterpret_cast<const cx<T> *>(base + a.off_cb);
-            double *red = reinterpret_cast<double *>(base + a.off_red);
-            double *gp = red + nacc(NL);
-            double *tmp = gp + 3 * FM_MAX_LEGS;
-            constexpr int n = NL;
-            if (lane < 4 * n) {
-                double s = 0.0;
-                for (int b = 0; b < nband; ++b) {
-                    const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
-                    const cx<T> *src = reinterpret_cast<const cx<T> *>(wg_work + (size_t)b * wpw) + (size_t)lane * FM_WAVE + j * cap;
-                    for (int i0 = 0; i0 < nb; i0 += 8) {   // 8 loads in flight, summed in order
-                        cx<T> d[8];
+    }
+}
+
+// The per-path tail of the fused adjoint, one wavefront per path, from the workspace regions k_adjoint left:
+// dL/dcchi (the fine-grid adjoint, or the given gcchi), pbc = conj(dL/dptrac) and the cfac terms, Re(pbc D) per
+// slot summed over the energies (FP64, a fixed butterfly), the geometry adjoint and the output. Few registers and a
+// few KB of LDS per wavefront, so that several wavefronts share a SIMD and hide the latencies this per-path work
+// consists of. No workgroup barrier: a wavefront without a path returns.
+template <class T, int NL>
+__global__ __launch_bounds__(FM_WAVE * 4) void k_adjoint_tail(BArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wid = threadIdx.x / FM_WAVE, lane = lane_id();
+    const int s0 = blockIdx.x * 4 + wid;
+    if (s0 >= a.npath_bucket) return;
+    const fm_edge &ed = a.ed;
+    const int E = ed.E;
+    constexpr int n = NL, NS = 4 * NL;
+    unsigned char *base = smem + (size_t)wid * a.path_bytes;
+    PathLDS &g = *reinterpret_cast<PathLDS *>(base);
+    cx<T> *cc = reinterpret_cast<cx<T> *>(base + a.off_cc);
+    T *amp = reinterpret_cast<T *>(base + a.off_amp);
+    double *ang = reinterpret_cast<double *>(base + a.off_ang);
+    cx<T> *cb = reinterpret_cast<cx<T> *>(base + a.off_cb);
+    double *red = reinterpret_cast<double *>(base + a.off_red);
+    double *gp = red + nacc(NL);
+    double *tmp = gp + 3 * FM_MAX_LEGS;
+    const int p = a.order ? a.order[a.first + s0] : a.first + s0;
+    int ei;
+    load_path(a.src, p, g, ei, NL);
+    const cx<T> *wp = reinterpret_cast<const cx<T> *>(a.work) + (size_t)s0 * a.dstride;
+    for (int e = lane; e < E; e += FM_WAVE) cc[e] = wp[e];
+    wave_sync();
+    double rb = 0.0;
+    if (a.src.mode == 1) {
+        amp_phase<T>(E, cc, amp, ang);
+        const double *gsr = a.gseg ? a.gseg + (size_t)a.segment[p] * a.nk : nullptr;
+        const T *gpr = a.gpath ? reinterpret_cast<const T *>(a.gpath) + (size_t)p * a.nk : nullptr;
+        rb = fine_grid_adjoint<T>(E, a.G.jq + (size_t)a.edge_id * a.nk, a.G.fq + (size_t)a.edge_id * a.nk, a.G.q,
+                                  a.G.qorder, a.G.qstart + (size_t)a.edge_id * a.G.qstride, g.reff * FM_BOHR,
+                                  a.src.P.sigma2 ? a.src.P.sigma2[p] : 0.0, gsr, a.src.P.deg[p], gpr, cc, amp, ang, cb,
+                                  a.nk, a.off_qs ? reinterpret_cast<double *>(base + a.off_qs) : nullptr,
+                                  a.G.S ? a.G.S + (size_t)a.edge_id * a.nk * a.G.sstride : nullptr, a.G.sstride);
+    } else {
+        const T *gin = reinterpret_cast<const T *>(a.gcchi) + (size_t)p * E * 2;
+        for (int e = lane; e < E; e += FM_WAVE) cb[e] = mk<T>(gin[2 * e], gin[2 * e + 1]);
+        rb = a.greff ? a.greff[p] : 0.0;
+    }
+    wave_sync();
+    // pbc and the two cfac terms: d cfac / d r_j = a1 - a2 / r_j with a1 = sum -Im(w (ck - xk)), a2 = sum Re w
+    double s1 = 0.0, s2 = 0.0;
+    for (int e = lane; e < E; e += FM_WAVE) {
+        const cx<T> cbar = cb[e];
+        const cx<T> w = conj(cbar) * cc[e];
+        const cx<T> wdk = w * mk<T>((T)ed.ckmxk[2 * e], (T)ed.ckmxk[2 * e + 1]);
+        s1 += (double)(-wdk.i);
+        s2 += (double)w.r;
+        cb[e] = conj(cbar * conj(cfac_at<T>(ed, g, e, NL)));
+    }
+    // Re(pbc D) per slot, the lanes over the energies with all the slots' loads in flight
+    double acc[NS];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) d[u] = i0 + u < nb ? src[i0 + u] : mk<T>(T(0), T(0));
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    for (int e = lane; e < E; e += FM_WAVE) {
+        const cx<T> q = cb[e];
+        cx<T> d[NS];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            if (i0 + u < nb) {
-                                const cx<T> q = pbc[e0 + i0 + u];
-                                s += (double)(q.r * d[u].r - q.i * d[u].i);
-                            }
-                    }
-                }
-                red[lane] = s;
-            } else if (lane < 4 * n + 2) {
-                double s = 0.0;
-                for (int e = 0; e < E; ++e) s += lane == 4 * n ? (double)amp[e] : ang[e];
-                red[lane] = s;
-            }
-            wave_sync();
-            if (lane < 3 * n) {
-                const int kind = lane / n, jg = lane % n + 1;
-                double v;
-                if (kind == 0)
-                    v = red[GK_X * n + jg - 1] + red[GK_Y * n + jg - 1] + red[4 * n] - red[4 * n + 1] / g.r[jg] +
-                        0.5 * FM_BOHR * rbar[j];
-                else if (kind == 1)
-                    v = red[GK_B * n + jg - 1];
-                else
-                    v = red[GK_H * n + jg - 1];
-                gp[lane] = v;
-            }
-            wave_sync();
-            const int ndir = a.src.mode == 0 ? 3 * n : 3 * (n - 1);
-            double gout = geometry_adjoint(g, n, a.src.mode, gp, reinterpret_cast<double *>(base + a.off_gs));
-            if (lane < ndir) {
-                if (!g.ok) gout = __builtin_nan("");
-                tmp[lane] = gout;
-            }
-            wave_sync();
-            const int p = pidx[j];
-            double *out = a.gcoord + (size_t)p * a.nslot * 3;
-            if (a.src.mode == 0) {
-                if (lane < 3 * n) out[lane] = gout;
-            } else {
-                if (lane < 3 * (n - 1)) out[3 + lane] = gout;
-                if (lane < 3) {
-                    double s = 0.0;
-                    for (int k = 0; k < n - 1; ++k) s += tmp[3 * k + lane];
-                    out[lane] = -s;
-                }
-                if (lane >= 3 * n && lane < 3 * a.nslot) out[lane] = 0.0;
-            }
-            wave_sync();
+        for (int k = 0; k < NS; ++k) d[k] = wp[(size_t)(1 + k) * E + e];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] += (double)(q.r * d[k].r - q.i * d[k].i);
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = wave_sum(acc[k]);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) red[k] = acc[k];
+        red[NS] = s1;
+        red[NS + 1] = s2;
+    }
+    wave_sync();
+    if (lane < 3 * n) {
+        const int kind = lane / n, jg = lane % n + 1;
+        double v;
+        if (kind == 0)
+            v = red[GK_X * n + jg - 1] + red[GK_Y * n + jg - 1] + red[NS] - red[NS + 1] / g.r[jg] + 0.5 * FM_BOHR * rb;
+        else if (kind == 1)
+            v = red[GK_B * n + jg - 1];
+        else
+            v = red[GK_H * n + jg - 1];
+        gp[lane] = v;
+    }
+    wave_sync();
+    const int ndir = a.src.mode == 0 ? 3 * n : 3 * (n - 1);
+    double gout = geometry_adjoint(g, n, a.src.mode, gp, reinterpret_cast<double *>(base + a.off_gs));
+    if (lane < ndir) {
+        if (!g.ok) gout = __builtin_nan("");
+        tmp[lane] = gout;
+    }
+    wave_sync();
+    double *out = a.gcoord + (size_t)p * a.nslot * 3;
+    if (a.src.mode == 0) {
+        if (lane < 3 * n) out[lane] = gout;
+    } else {
+        if (lane < 3 * (n - 1)) out[3 + lane] = gout;
+        if (lane < 3) {
+            double sum = 0.0;
+            for (int k = 0; k < n - 1; ++k) sum += tmp[3 * k + lane];
+            out[lane] = -sum;
         }
-        __syncthreads();
+        if (lane >= 3 * n && lane < 3 * a.nslot) out[lane] = 0.0;
     }
 }
 
@@ -1497,38 +1503,88 @@ hipError_t launch_fused_nl(const BArgs &a, long blocks, size_t shm, hipStream_t 
     return hipGetLastError();
 }
 
-// The fused adjoint (multiple scattering of 3..FM_FUSED_MAXLEG legs, single scattering): the same LDS layout as
-// k_backward, a workspace slice per band holding 4 nleg complex derivative slots per lane. Single scattering keeps
-// little in registers: two workgroups per CU (FP32).
+// LDS of one path in k_adjoint: the geometry and the rotation tables with their beta derivatives.
+template <class T, int SET> __host__ void layout_sweep(BArgs &a) {
+    size_t o = align16(sizeof(PathLDS));
+    const size_t tab = SET >= FM_SET_SS ? 0 : align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
+    a.off_W = o; o += tab;
+    a.off_Wd = o; o += tab;
+    a.path_bytes = o;
+}
+
+// LDS of one wavefront (one path) of k_adjoint_tail.
+template <class T> __host__ void layout_tail(BArgs &a, const fm_edge &ed) {
+    size_t o = align16(sizeof(PathLDS));
+    a.off_cc = o; o += align16(sizeof(cx<T>) * ed.E);
+    a.off_amp = o; o += align16(sizeof(T) * ed.E);
+    a.off_ang = o; o += align16(sizeof(double) * ed.E);
+    a.off_cb = o; o += align16(sizeof(cx<T>) * ed.E);
+    a.off_red = o; o += align16(sizeof(double) * (nacc(a.nlegmax) + 6 * FM_MAX_LEGS));
+    const size_t qs = sizeof(double) * 3 * (size_t)a.nk;
+    a.off_qs = 0;
+    if (a.src.mode == 1 && qs > 0 && qs <= FM_QS_MAX) { a.off_qs = o; o += align16(qs); }
+    a.off_gs = o; o += align16(sizeof(double) * 3 * (size_t)a.nlegmax * a.nlegmax);
+    a.path_bytes = o;
+}
+
+template <class T, int NL> hipError_t launch_tail(const BArgs &a, hipStream_t st) {
+    const size_t shm = a.path_bytes * 4;
+    if (shm > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_adjoint_tail<T, NL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_adjoint_tail<T, NL>), dim3((unsigned)((a.npath_bucket + 3) / 4)), dim3(FM_WAVE * 4), shm, st, a);
+    return hipGetLastError();
+}
+
+constexpr size_t FM_DREGION_MAX = (size_t(1) << 31) - 4096;   // the sweep addresses its workspace with 32-bit offsets
+
+// The fused adjoint (multiple scattering of 3..FM_FUSED_MAXLEG legs, single scattering): k_adjoint's sweep, then
+// k_adjoint_tail, over chunks of the bucket of as many paths as the workspace holds regions for (one spare). Single
+// scattering keeps little in registers: two workgroups per CU (FP32).
 template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
-    layout<T, SET>(a, ed, true);
-    a.work_per_wave = ((size_t)4 * a.nlegmax * FM_WAVE * sizeof(cx<T>) + 255) & ~size_t(255);
+    const int nl = SET >= FM_SET_SS ? 2 : a.nlegmax;
+    a.dstride = (size_t)(4 * nl + 1) * ed.E;
+    const size_t per = a.dstride * sizeof(cx<T>);
+    const size_t room = a.work_bytes < FM_DREGION_MAX ? a.work_bytes : FM_DREGION_MAX;
+    const long chunk = (long)(room / per) - 1;
+    if (chunk < 1) return hipErrorOutOfMemory;
+    BArgs A = a, B = a;
+    layout_sweep<T, SET>(A);
     size_t budget = fm_lds_budget(sizeof(T) == 4 ? 1 : 3);
     if (SET >= FM_SET_SS && budget > FM_LDS_SHARE) budget = FM_LDS_SHARE;
-    int g = pick_gpaths(ed, a.path_bytes, budget - 256);
-    if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+    int g = pick_gpaths(ed, A.path_bytes, budget - 256);
+    if (g < 0) g = pick_gpaths(ed, A.path_bytes, FM_LDS_MAX - 256);
     if (g < 0) return hipErrorInvalidValue;
-    a.gp = 1 << g;
-    a.bp = ed.bands[g];
-    const size_t shm = a.path_bytes * a.gp;
-    const long slots = (long)(a.work_bytes / a.work_per_wave);
-    if (slots < a.bp.nband) return hipErrorInvalidValue;
-    long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
-    if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
-    if (blocks == 0) return hipSuccess;
-    if constexpr (SET >= FM_SET_SS) {
-        if (sizeof(T) == 4 && shm <= FM_LDS_SHARE - 256) return launch_fused_nl<T, SET, 2, 2>(a, blocks, shm, st);
-        return launch_fused_nl<T, SET, 2, 1>(a, blocks, shm, st);
-    } else {
-        switch (a.nlegmax) {
-            case 3: return launch_fused_nl<T, SET, 3>(a, blocks, shm, st);
-            case 4: return launch_fused_nl<T, SET, 4>(a, blocks, shm, st);
-            case 5: return launch_fused_nl<T, SET, 5>(a, blocks, shm, st);
-            case 6: return launch_fused_nl<T, SET, 6>(a, blocks, shm, st);
+    A.gp = 1 << g;
+    A.bp = ed.bands[g];
+    const size_t shm = A.path_bytes * A.gp;
+    layout_tail<T>(B, ed);
+    if (B.path_bytes * 4 > FM_LDS_MAX) return hipErrorInvalidValue;
+    for (long c0 = 0; c0 < a.npath_bucket; c0 += chunk) {
+        const int cnt = (int)(a.npath_bucket - c0 < chunk ? a.npath_bucket - c0 : chunk);
+        A.first = B.first = a.first + (int)c0;
+        A.npath_bucket = B.npath_bucket = cnt;
+        A.dbytes = B.dbytes = (size_t)(cnt + 1) * per;
+        const long blocks = (cnt + A.gp - 1) / A.gp;
+        hipError_t e;
+        if constexpr (SET >= FM_SET_SS) {
+            if (sizeof(T) == 4 && shm <= FM_LDS_SHARE - 256) e = launch_fused_nl<T, SET, 2, 2>(A, blocks, shm, st);
+            else e = launch_fused_nl<T, SET, 2, 1>(A, blocks, shm, st);
+            if (e == hipSuccess) e = launch_tail<T, 2>(B, st);
+        } else {
+            switch (a.nlegmax) {
+                case 3: e = launch_fused_nl<T, SET, 3>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 3>(B, st); break;
+                case 4: e = launch_fused_nl<T, SET, 4>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 4>(B, st); break;
+                case 5: e = launch_fused_nl<T, SET, 5>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 5>(B, st); break;
+                case 6: e = launch_fused_nl<T, SET, 6>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 6>(B, st); break;
+                default: e = hipErrorInvalidValue;
+            }
         }
-        return hipErrorInvalidValue;
+        if (e != hipSuccess) return e;
     }
+    return hipSuccess;
 }
 
 // Multiple scattering of more than FM_FUSED_MAXLEG legs: pass A stores the prefix products in the workspace, pass B
@@ -1629,6 +1685,7 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
     if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_cchi_backward: a path's tables do not fit one workgroup's LDS");
+    if (e == hipErrorOutOfMemory) return set_err(FM_EINVAL, "fm_cchi_backward: workspace too small for one path");
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi_backward launch");
 }
 
@@ -1724,6 +1781,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
         if (e == hipErrorInvalidValue)
             return set_err(FM_EINVAL, "fm_chi_paths_backward: the bucket's paths do not fit one workgroup's LDS");
+        if (e == hipErrorOutOfMemory) return set_err(FM_EINVAL, "fm_chi_paths_backward: workspace too small for one path");
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths_backward launch");
     }
     if (grad_pos && natom > 0) {

@@ -203,7 +203,8 @@ class PathPlan:
         self.atom_ptr_t = i32(np.cumsum(ptr))
         self.atom_refs_t = i32(ref_id[o])
         self.gcoord = torch.empty(Pn, nmax, 3, dtype=torch.float64, device=dev)
-        self.workspace = N.backward_workspace(self.precision, nmax, Pn, dev) if Pn else None
+        big = int(np.diff(self.buckets.offset).max()) if Pn else 0
+        self.workspace = N.backward_workspace(self.precision, nmax, big, dev, max(T.E for T in self.tables)) if Pn else None
         self.chi_path = torch.empty(Pn, nk, dtype=self.rdtype, device=dev)
         self.reff = torch.empty(Pn, dtype=torch.float64, device=dev)
         self._h_off = (N._i32 * len(self.buckets.offset))(*self.buckets.offset.tolist())

@@ -122,7 +122,7 @@ class GPUPathStage:
         gr = None if greff is None else greff.to(torch.float64).contiguous()
         out = torch.zeros(B, nleg, 3, dtype=torch.float64, device=self.device)
         if B:
-            ws = N.backward_workspace(self.precision, nleg, B, self.device)
+            ws = N.backward_workspace(self.precision, nleg, B, self.device, self.E)
             N.check(N.lib().fm_cchi_backward(self._edge.struct, self.precision, self.iorder, B, nleg, c.data_ptr(),
                                              ip.data_ptr(), gc.data_ptr(), N.ptr(gr), out.data_ptr(), ws.data_ptr(),
                                              ws.numel(), N.stream_of(self.device)), "fm_cchi_backward")

@@ -112,8 +112,11 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
                      void *workspace, size_t workspace_bytes, void *stream);
 
 /* Bytes of the workspace the adjoints need for `band_slots` energy bands of paths of at most nlegmax legs in flight
- * at once (each lane keeps its scattering matrices, prefix products and gradient terms there; a workgroup holds one
- * slot per band of its plan, at most 16, and a launch uses as many workgroups as the workspace holds). */
+ * at once (paths of 7-9 legs: each lane keeps its scattering matrices, prefix products and gradient terms there; a
+ * workgroup holds one slot per band of its plan, at most 16, and a launch uses as many workgroups as the workspace
+ * holds). The adjoint of 2-6 legs keeps (4 nleg + 1) E complex per path (cchi and the derivative slots of every
+ * energy) for as many paths of a bucket at a time as the workspace holds regions for, less one spare: larger buckets
+ * run in chunks. Any workspace of at least fm_backward_workspace_bytes(precision, nlegmax, 16) bytes works. */
 size_t fm_backward_workspace_bytes(int32_t precision, int32_t nlegmax, int32_t band_slots);
 
 /* GPUPathStage.chi_fine with the linear ("feff") convention (src/feffgpu/stage.py:365-388): chi [npath][nk] =
