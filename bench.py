@@ -281,7 +281,7 @@ def main():
     kset = int(plan32.buckets.kind[bdom]) if nl > 2 else 3 + plan32.tables[int(plan32.buckets.edge[bdom])].ilinit - 1
     roof = dict(bound="mfma", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak,
                 traffic=measured_traffic("k_adjoint") if (kset, nl) == (0, 4) else None,
-                kernel=f"k_adjoint<float,{kset},{nl},*> ({what})", launch_ms=t_bwd,
+                kernel=f"k_adjoint<float,{kset},{nl},*> + k_adjoint_tail<float,{nl}> ({what})", launch_ms=t_bwd,
                 flops_per_launch=2 * b_work[bdom], note=note)
     roof_fwd = dict(achieved=ach_f, peak=peak, unit="TFLOP/s", frac=ach_f / peak,
                     kernel=f"k_forward<float,{kset}> ({what})", launch_ms=t_fwd,

@@ -1294,16 +1294,17 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
     const int sbytes = E * (int)sizeof(cx<T>);
 
     for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
-        // ---- 1. setup: one wavefront per path
-        for (int j = wid; j < G; j += 4) {
+        {   // ---- 1. setup: the wavefront's paths j = wid, wid + 4 at once, a 16-lane group each (G <= 8)
+            const int j = wid + 4 * (lane >> 4);
+            const bool act = j < G;
             const int slot = blk * G + j;
-            const int s0 = slot < a.npath_bucket ? slot : blk * G;
+            const int s0 = act && slot < a.npath_bucket ? slot : blk * G;
             const int p = a.order ? a.order[a.first + s0] : a.first + s0;
-            PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)j * a.path_bytes);
+            PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)(act ? j : 0) * a.path_bytes);
             int ei;
-            load_path(a.src, p, g, ei, NL);
+            load_path_group(a.src, p, g, ei, NL, lane & 15, act);
         }
-        if constexpr (SET < FM_SET_SS) build_tables<T, SET, true>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
+        if constexpr (SET < FM_SET_SS) build_tables<T, SET, true, 16>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
         __syncthreads();
         // ---- 2. the fused sweep over this wavefront's bands
         const int sj = blk * G + jl;
@@ -1570,7 +1571,8 @@ template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
         const long blocks = (cnt + A.gp - 1) / A.gp;
         hipError_t e;
         if constexpr (SET >= FM_SET_SS) {
-            if (sizeof(T) == 4 && shm <= FM_LDS_SHARE - 256) e = launch_fused_nl<T, SET, 2, 2>(A, blocks, shm, st);
+            // K: two workgroups per CU; L and M (27 and 21 representatives) would spill at that register budget
+            if (sizeof(T) == 4 && SET == FM_SET_SS && shm <= FM_LDS_SHARE - 256) e = launch_fused_nl<T, SET, 2, 2>(A, blocks, shm, st);
             else e = launch_fused_nl<T, SET, 2, 1>(A, blocks, shm, st);
             if (e == hipSuccess) e = launch_tail<T, 2>(B, st);
         } else {

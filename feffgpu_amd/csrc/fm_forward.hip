@@ -84,17 +84,31 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
     const PathLDS &gl = *reinterpret_cast<const PathLDS *>(lbase);
     const T *Wl = reinterpret_cast<const T *>(lbase + off_W);
     for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
-        // ---- setup: one wavefront per path
-        for (int j = wid; j < G; j += 4) {
+        // ---- setup. Single scattering (no tables): the wavefront's paths j = wid, wid + 4 at once, a 16-lane group
+        // each (G <= 8); multiple scattering: one path after the other (measured faster there, the two workgroups of
+        // a CU overlapping their setups)
+        if constexpr (SET >= FM_SET_SS) {
+            const int j = wid + 4 * (lane >> 4);
+            const bool act = j < G;
             const int slot = blk * G + j;
-            const bool valid = slot < a.npath_bucket;
+            const bool valid = act && slot < a.npath_bucket;
             const int s0 = valid ? slot : blk * G;
             const int p = a.order ? a.order[a.first + s0] : a.first + s0;
-            unsigned char *base = smem + (size_t)j * a.path_bytes;
-            PathLDS &g = *reinterpret_cast<PathLDS *>(base);
+            PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)(act ? j : 0) * a.path_bytes);
             int ei;
-            load_path(a.src, p, g, ei, nl);
-            if (lane == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
+            load_path_group(a.src, p, g, ei, nl, lane & 15, act);
+            if (act && (lane & 15) == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
+        } else {
+            for (int j = wid; j < G; j += 4) {
+                const int slot = blk * G + j;
+                const bool valid = slot < a.npath_bucket;
+                const int s0 = valid ? slot : blk * G;
+                const int p = a.order ? a.order[a.first + s0] : a.first + s0;
+                PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)j * a.path_bytes);
+                int ei;
+                load_path(a.src, p, g, ei, nl);
+                if (lane == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
+            }
         }
         if constexpr (SET < FM_SET_SS) build_tables<T, SET, false, 1>(ed, smem, a.path_bytes, off_W, 0, wid, 4, G, nl, ltab);
         __syncthreads();

@@ -31,6 +31,7 @@ struct PathLDS {
     double r[FM_MAX_LEGS + 1], beta[FM_MAX_LEGS + 1], alpha[FM_MAX_LEGS + 1], gamma[FM_MAX_LEGS + 1];
     double eta[FM_MAX_LEGS + 1];
     double c1[FM_MAX_LEGS + 1], s1[FM_MAX_LEGS + 1], c2[FM_MAX_LEGS + 1], s2[FM_MAX_LEGS + 1];  // cos/sin eta, 2 eta
+    double cb[FM_MAX_LEGS + 1], sb[FM_MAX_LEGS + 1], ch[FM_MAX_LEGS + 1], sh[FM_MAX_LEGS + 1];  // cos/sin beta, beta/2
     int ipot[FM_MAX_LEGS + 1];       // potential of path atom j (0 and n: absorber)
     double reff, R, prod_r;           // bohr
     int n, ok, pad_[2];               // ok = 0: placeholder for a path whose leg count is not the launch's
@@ -110,17 +111,18 @@ __device__ inline void path_point(const fm_paths &P, int p, int j, double (&v)[3
     for (int c = 0; c < 3; ++c) v[c] /= FM_BOHR;
 }
 
-// Fills g (geometry, potentials) for path slot `p`; all lanes call it. Every path of a launch has nuni legs (the
-// kernels loop over the legs uniformly): a path-list entry with another leg count is replaced by a fixed
-// placeholder path of nuni legs with g.ok = 0, so that every index stays in bounds; the kernels write NaN for it.
-__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni) {
-    const int lane = lane_id();
+// Fills g (geometry, potentials) for path slot `p`; all lanes of the wavefront call it, those with act set work on
+// this path as lanes ln = 0..15 of a group (several groups of a wavefront fill several paths at once). Every path
+// of a launch has nuni legs (the kernels loop over the legs uniformly): a path-list entry with another leg count is
+// replaced by a fixed placeholder path of nuni legs with g.ok = 0, so that every index stays in bounds; the kernels
+// write NaN for it.
+__device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni, int lane, bool act) {
     int n;
     bool ok = true;
     if (a.mode == 0) {
         n = nuni;
         edge_idx = 0;
-        if (lane < n) {
+        if (act && lane < n) {
             const double *c = a.coords + ((size_t)p * n + lane) * 3;
             double x = c[0] / FM_BOHR, y = c[1] / FM_BOHR, z = c[2] / FM_BOHR;
             // absorber (lane 0) is the path's first point; the stage places it at rat[0] and rat[n]
@@ -133,13 +135,13 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
         edge_idx = a.P.edge ? a.P.edge[p] : 0;
         const int nmax = a.P.nmax;
         if (!ok) {
-            if (lane < n) {
+            if (act && lane < n) {
                 g.rat[lane][0] = 4.0 * lane;
                 g.rat[lane][1] = 1.5 * (lane & 1);
                 g.rat[lane][2] = 0.5 * lane;
                 g.ipot[lane] = 0;
             }
-        } else if (lane < n) {
+        } else if (act && lane < n) {
             double v[3];
             path_point(a.P, p, lane, v);
 #pragma unroll
@@ -148,11 +150,11 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
         }
     }
     wave_sync();
-    if (lane < 3) g.rat[n][lane] = g.rat[0][lane];
-    if (lane == 0) { g.ipot[n] = g.ipot[0]; g.n = n; g.ok = ok ? 1 : 0; }
+    if (act && lane < 3) g.rat[n][lane] = g.rat[0][lane];
+    if (act && lane == 0) { g.ipot[n] = g.ipot[0]; g.n = n; g.ok = ok ? 1 : 0; }
     wave_sync();
     // one lane per leg
-    if (lane < n) {
+    if (act && lane < n) {
         LegGeo lg = leg_geometry(g.rat, lane + 1, n);
         g.r[lane + 1] = lg.r;
         g.beta[lane + 1] = lg.beta;
@@ -160,7 +162,7 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
         g.gamma[lane + 1] = lg.gamma;
     }
     wave_sync();
-    if (lane < n) {
+    if (act && lane < n) {
         const int J = lane + 1;
         double eta = (J == 1 ? g.alpha[n] : g.alpha[J - 1]) + g.gamma[J];
         g.eta[J] = eta;
@@ -169,8 +171,13 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
         g.c1[J] = c; g.s1[J] = s;
         sincos(2.0 * eta, &s, &c);
         g.c2[J] = c; g.s2[J] = s;
+        const double be = g.beta[J];
+        sincos(be, &s, &c);
+        g.cb[J] = c; g.sb[J] = s;
+        sincos(0.5 * be, &s, &c);
+        g.ch[J] = c; g.sh[J] = s;
     }
-    if (lane == 0) {
+    if (act && lane == 0) {
         double R = 0.0, pr = 1.0;
         for (int j = 1; j <= n; ++j) { R += g.r[j]; pr *= g.r[j]; }
         g.R = R; g.reff = 0.5 * R; g.prod_r = pr;
@@ -179,10 +186,15 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
     return n;
 }
 
+__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni) {
+    return load_path_group(a, p, g, edge_idx, nuni, lane_id(), true);
+}
+
 // The recursion coefficients of B consecutive rows il0.. of one (m1, m2) (mm, nn as build_tables forms them),
 // fetched together so that their load latencies overlap: from l - 1 to l for l >= 3, and the normalisation.
 template <int B> struct RotCoef {
-    double c0[B], c1[B], c2[B], nr[B];
+    typedef double vec __attribute__((ext_vector_type(B)));   // vector values, not arrays: kept in registers
+    vec c0, c1, c2, nr;
     __device__ __forceinline__ void fetch(const fm_edge &ed, int il0, int rows, int mm, int nn) {
 #pragma unroll
         for (int u = 0; u < B; ++u) {
@@ -206,8 +218,8 @@ template <int B> struct RotCoef {
 // il = 4); with WITH_D also Wd = the same with d/dbeta d. Three-term recursion in l for fixed (m1, m2)
 // (src/feffgpu/stage.py:77-92) seeded by the closed forms at l <= 2, in FP64, differentiated:
 // d'^l = (P c - Q) d'^{l-1} - P sin(beta) d^{l-1} - R d'^{l-2}. The (path, leg, representative) tasks of all the
-// wavefront's paths are spread over its lanes. The coefficients of BATCH rows are fetched at a time (the forward
-// kernel, at its register limit, takes 1).
+// wavefront's paths are spread over its lanes; cos/sin of beta and beta/2 come from the path's geometry. The
+// coefficients of BATCH rows are fetched at a time, as vector values held in registers.
 template <class T, int SET, bool WITH_D, int BATCH = 8, int ILP = 1>
 __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path_bytes, size_t off_W, size_t off_Wd,
                              int j0, int jstep, int G, int n, int ltab) {
@@ -234,8 +246,7 @@ __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path
 #pragma unroll
             for (int r = 0; r < FM_NREP; ++r)
                 if (r == k) { m1[q] = PD.m1[r]; m2[q] = PD.m2[r]; }
-            const double be = g.beta[s + 1];
-            c[q] = cos(be); sn[q] = sin(be); c2[q] = cos(0.5 * be); s2[q] = sin(0.5 * be);
+            c[q] = g.cb[s + 1]; sn[q] = g.sb[s + 1]; c2[q] = g.ch[s + 1]; s2[q] = g.sh[s + 1];
             Ws[q] = reinterpret_cast<T *>(base + off_W) + (size_t)s * ltab * FM_NREP;
             Wds[q] = reinterpret_cast<T *>(base + off_Wd) + (size_t)s * ltab * FM_NREP;
             kk[q] = k;
@@ -251,9 +262,8 @@ __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path
 #pragma unroll
             for (int q = 0; q < ILP; ++q) rc[q].fetch(ed, il0, rows[q], mm[q], nn[q]);
 #pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
+            for (int u = 0; u < BATCH; ++u) {   // no early exit: the batch stays in registers
                 const int il = il0 + u, l = il - 1;
-                if (il > rmax) break;
 #pragma unroll
                 for (int q = 0; q < ILP; ++q) {
                     if (il > rows[q]) continue;
