@@ -287,6 +287,15 @@ def main():
                     kernel=f"k_forward<float,{kset}> ({what})", launch_ms=t_fwd,
                     flops_per_launch=b_work[bdom])
 
+    # the misfit after the sum: R-factor of every frame and its gradient (fm_r_factor), the alternative loss_grad of
+    # the step; timed on its own
+    from feffgpu_amd.transform import FitWindow, RFactorLoss
+    rloss = RFactorLoss(target, q, FitWindow(3.0, 16.0, 1.0, 2, 1.0, 5.0), device=dev)
+    seg32 = plan32.forward(pos_t)[0].clone()
+    t_rf = launch_ms(lambda: rloss(seg32))
+    misfit = dict(kernel="k_r_factor", segments=int(nseg), r_points=rloss.nm, launch_ms=t_rf,
+                  note="R-factor per frame and d R / d chi_seg in one launch (FP64)")
+
     out = dict(metric=METRIC, value=units / (ms_fb32 * 1e-3), unit="paths·k-pts/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=ms_fb32, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="fp32 (fp64 geometry and phase)", data="synthetic",
@@ -305,7 +314,7 @@ def main():
                parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
                                   backward_ms=policy["backward"]),
-               roofline=roof, roofline_forward=roof_fwd)
+               roofline=roof, roofline_forward=roof_fwd, misfit=misfit)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, pl, q, cell, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -46,6 +46,11 @@ class FmGrid(ctypes.Structure):
                 ("S", _vp), ("sstride", _i32)]
 
 
+class FmWindow(ctypes.Structure):
+    _fields_ = [("kmin", ctypes.c_double), ("kmax", ctypes.c_double), ("dk", ctypes.c_double),
+                ("kweight", _i32), ("rmin", ctypes.c_double), ("rmax", ctypes.c_double)]
+
+
 def grid_runs(q, jq_rows, E_list):
     """qorder (q indices in ascending q) and qstart [nedge][stride]: for every edge, the first position in qorder
     whose lower interpolation index is >= e, for e = 0..E (the runs the adjoint of the fine grid gathers)."""
@@ -101,6 +106,8 @@ def lib():
                                          _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
                                          ctypes.POINTER(_i32), ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, _vp,
                                          ctypes.c_size_t, _vp]
+        L.fm_r_factor.argtypes = [_i32, _i32, ctypes.c_double, _i32, ctypes.POINTER(FmWindow), _i32, _i32, _vp,
+                                  ctypes.c_double, _vp, _i32, _vp, _vp, _vp]
         L.fm_set_lds_budget.restype = _i32
         L.fm_set_lds_budget.argtypes = [_i32, _i32]
         _lib = L

@@ -19,7 +19,8 @@ import torch
 
 class CapturedStep:
     """plan: a PathPlan. natom: rows of the position array. adjoint: capture the backward too. loss_grad: maps
-    chi_seg (nseg, NK) float64 to dL/dchi_seg inside the graph (torch ops only); without it the adjoint reads the
+    chi_seg (nseg, NK) float64 to dL/dchi_seg inside the graph (torch ops, or feffgpu_amd.transform.RFactorLoss:
+    launches on the current stream that allocate nothing once warmed up); without it the adjoint reads the
     static buffer `gseg`. warmup: eager runs before capture (they also set the kernels' LDS attributes)."""
 
     def __init__(self, plan, natom: int, adjoint: bool = True, loss_grad: Optional[Callable] = None, warmup: int = 2):

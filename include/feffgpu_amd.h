@@ -175,6 +175,26 @@ int fm_chi_paths_plain(const fm_edge *edges, int32_t nedge, const fm_paths *path
                        const int32_t *seg_paths, double *chi_seg, double *chi_path, double *reff, void *workspace,
                        size_t workspace_bytes, void *stream);
 
+/* The k window and R range of a comparison (FitWindow, src/feffgpu/exafs/transform.py:30-62): Hanning sills
+ * centered on kmin and kmax, dk wide (dk <= 0: the rectangular window kmin..kmax); k^kweight chi(k) is transformed. */
+typedef struct fm_window {
+    double kmin, kmax, dk;
+    int32_t kweight;             /* 0..16 */
+    double rmin, rmax;           /* A; the caller turns them into the range m0 .. m0 + nm - 1 of R_m = m pi / (nfft kstep) */
+} fm_window;
+
+#define FM_RF_NFFT_MAX 4096
+
+/* The misfit after the path sum, r_factor_torch (src/feffgpu/exafs/transform.py:130-145) and compare.r_factor
+ * (src/feffgpu/compare.py:17-27), per segment: chi [nseg][nk] on the grid k_j = j kstep (times `scale`, e.g. 1 / the
+ * number of absorbers), data [nseg][nk] (data_stride nk) or one spectrum [nk] (data_stride 0). The transform of
+ * k^w chi window, zero-padded to nfft (a power of two <= 4096, >= nk), times kstep / sqrt(pi), is taken at
+ * m = m0 .. m0 + nm - 1 (< nfft / 2). rfac [nseg] = sum_m |M_m - D_m|^2 / sum_m |D_m|^2; gchi [nseg][nk] (or NULL) =
+ * d rfac / d chi. FP64, one workgroup per segment, no allocation (capturable). */
+int fm_r_factor(int32_t nseg, int32_t nk, double kstep, int32_t nfft, const fm_window *w, int32_t m0, int32_t nm,
+                const double *chi, double scale, const double *data, int32_t data_stride, double *rfac, double *gchi,
+                void *stream);
+
 /* Launch policy (process-wide): the LDS in KiB (16..160) that a workgroup may take when the number of paths per
  * workgroup is chosen, for kernel 0 (FP32 forward calls), 1 (FP32 adjoints), 2 (FP64 forward), 3 (FP64 adjoints);
  * defaults 80 (two FP32 forward workgroups per CU), then 160. Set by feffgpu_amd.tune from timings. Returns the

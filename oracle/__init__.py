@@ -190,3 +190,32 @@ def chi_paths(phases, pos, atoms, nleg, absorber, ipots, deg, q, segment=None, n
                                len(q), _p(q, _dp), _p(cellv, _dp), ncell, int(iorder), ROTMODE[rotation], nthr, _p(chi_seg, _dp),
                                _p(chi_path, _dp), _p(reff, _dp)))
     return chi_seg, chi_path, reff
+
+
+# ------------------------------------------------------------------------------------------------ misfit after the sum
+def hanning(k, kmin, kmax, dk=1.0):
+    """xraylarch's Hanning window (src/feffgpu/compare.py:6-14, src/feffgpu/exafs/transform.py:71-94): sills centered
+    on kmin and kmax, dk wide; dk <= 0 the rectangular window."""
+    k = np.asarray(k, float)
+    if dk <= 0:
+        return ((k >= kmin) & (k <= kmax)).astype(float)
+    a = np.clip((k - (kmin - dk / 2)) / dk, 0.0, 1.0)
+    b = np.clip((k - (kmax - dk / 2)) / dk, 0.0, 1.0)
+    return 0.5 * (1.0 - np.cos(np.pi * a)) * 0.5 * (1.0 + np.cos(np.pi * b))
+
+
+def chi_r(chi, kstep, kmin, kmax, dk, kweight, nfft=2048):
+    """Complex chi(R) at R_m = m pi / (nfft kstep), m < nfft / 2, of chi (..., NK) on k_j = j kstep: numpy's FFT of
+    k^w chi window zero-padded to nfft, times kstep / sqrt(pi) (src/feffgpu/exafs/transform.py:104-122)."""
+    chi = np.asarray(chi, float)
+    k = kstep * np.arange(chi.shape[-1])
+    x = chi * k ** kweight * hanning(k, kmin, kmax, dk)
+    return np.fft.fft(x, n=nfft, axis=-1)[..., : nfft // 2] * kstep / np.sqrt(np.pi)
+
+
+def r_factor(chi, data, kstep, kmin, kmax, dk, kweight, m0, nm, nfft=2048, scale=1.0):
+    """Per row of chi (S, NK): sum |M - D|^2 / sum |D|^2 over R points m0 .. m0 + nm - 1, M the transform of
+    scale * chi, D of data ((NK,) or (S, NK)) (src/feffgpu/exafs/transform.py:130-145, src/feffgpu/compare.py:17-27)."""
+    M = chi_r(scale * np.atleast_2d(chi), kstep, kmin, kmax, dk, kweight, nfft)[:, m0:m0 + nm]
+    D = chi_r(np.atleast_2d(data), kstep, kmin, kmax, dk, kweight, nfft)[:, m0:m0 + nm]
+    return (np.abs(M - D) ** 2).sum(1) / (np.abs(D) ** 2).sum(1)

@@ -40,9 +40,10 @@ def test_struct_layout_matches_ctypes(tmp_path):
 #include "{HEADER}"
 #define F(S, m) printf("%s.%s %zu\\n", #S, #m, offsetof(S, m))
 int main(void) {{
-  printf("fm_edge %zu\\nfm_paths %zu\\nfm_grid %zu\\n", sizeof(fm_edge), sizeof(fm_paths), sizeof(fm_grid));
+  printf("fm_edge %zu\\nfm_paths %zu\\nfm_grid %zu\\nfm_window %zu\\n", sizeof(fm_edge), sizeof(fm_paths), sizeof(fm_grid),
+         sizeof(fm_window));
   F(fm_edge, lmax1); F(fm_edge, lmax); F(fm_edge, rotn); F(fm_paths, pos); F(fm_paths, cell_kind); F(fm_paths, cellinv);
-  F(fm_grid, q); F(fm_grid, fq);
+  F(fm_grid, q); F(fm_grid, fq); F(fm_grid, S); F(fm_grid, sstride); F(fm_window, kweight); F(fm_window, rmin);
   return 0;
 }}
 """)
@@ -52,10 +53,13 @@ int main(void) {{
     assert int(out["fm_edge"]) == ctypes.sizeof(N.FmEdge)
     assert int(out["fm_paths"]) == ctypes.sizeof(N.FmPaths)
     assert int(out["fm_grid"]) == ctypes.sizeof(N.FmGrid)
+    assert int(out["fm_window"]) == ctypes.sizeof(N.FmWindow)
     for key, (cls, field) in {"fm_edge.lmax1": (N.FmEdge, "lmax1"), "fm_edge.lmax": (N.FmEdge, "lmax"),
                               "fm_edge.rotn": (N.FmEdge, "rotn"), "fm_paths.pos": (N.FmPaths, "pos"),
                               "fm_paths.cell_kind": (N.FmPaths, "cell_kind"), "fm_paths.cellinv": (N.FmPaths, "cellinv"),
-                              "fm_grid.q": (N.FmGrid, "q"), "fm_grid.fq": (N.FmGrid, "fq")}.items():
+                              "fm_grid.q": (N.FmGrid, "q"), "fm_grid.fq": (N.FmGrid, "fq"), "fm_grid.S": (N.FmGrid, "S"),
+                              "fm_grid.sstride": (N.FmGrid, "sstride"), "fm_window.kweight": (N.FmWindow, "kweight"),
+                              "fm_window.rmin": (N.FmWindow, "rmin")}.items():
         assert int(out[key]) == getattr(cls, field).offset, key
 
 

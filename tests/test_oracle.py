@@ -213,3 +213,41 @@ def test_dense_fine_grid_matches_linear_and_spline_interpolates():
     assert np.abs(spline_matrix(kf, q) - Ssp).max() == 0.0
     jj, ff = interpolation(kf, q)
     assert np.array_equal(jj, j) and np.abs(ff - f).max() < 1e-15
+
+
+def test_misfit_oracle_pins():
+    """The oracle's window, transform and R-factor (oracle.hanning / chi_r / r_factor) against what fixes them: the
+    reference's closed form for a scaled spectrum ((1/0.8 - 1)^2 = 0.0625, as tests/test_mcp.py:83 of the reference
+    has it), the Hanning window at its sill centres, on its plateau and outside, the transform of a single sample
+    (normalisation kstep / sqrt(pi), the e^{-i} sign, the R grid), Parseval's identity over the half spectrum, and
+    the product's numpy form (feffgpu_amd.compare.r_factor, src/feffgpu/compare.py:17-27)."""
+    from feffgpu_amd.compare import r_factor as r_np
+    from feffgpu_amd.transform import FitWindow, r_range
+    kstep, N = 0.05, 2048
+    k = kstep * np.arange(361)
+    w = oracle.hanning(k, 3.0, 14.0, 1.0)
+    assert w[k < 2.5 - 1e-9].max() == 0.0 and w[k > 14.5 + 1e-9].max() == 0.0
+    assert np.abs(w[(k > 3.5 + 1e-9) & (k < 13.5 - 1e-9)] - 1.0).max() < 1e-15
+    assert np.abs(oracle.hanning(np.array([3.0, 14.0]), 3.0, 14.0, 1.0) - 0.5).max() < 1e-15
+    assert np.array_equal(oracle.hanning(k, 3.0, 14.0, 0.0), ((k >= 3.0) & (k <= 14.0)).astype(float))
+    rng = np.random.default_rng(11)
+    data = rng.normal(size=361)
+    assert abs(oracle.r_factor(data / 0.8, data, kstep, 3, 14, 1, 2, 33, 163)[0] - 0.0625) < 1e-12
+    j0 = 150                                   # k = 7.5, on the plateau
+    delta = np.zeros(361)
+    delta[j0] = 1.0
+    m = np.arange(N // 2)
+    want = kstep / np.sqrt(np.pi) * k[j0] ** 2 * np.exp(-2j * np.pi * j0 * m / N)
+    assert np.abs(oracle.chi_r(delta, kstep, 3, 14, 1, 2) - want).max() < 1e-13
+    x = k ** 3 * oracle.hanning(k, 2.0, 16.0, 2.0) * data
+    X0, XN = x.sum(), (x * (-1.0) ** np.arange(361)).sum()
+    lhs = (np.abs(oracle.chi_r(data, kstep, 2, 16, 2, 3)) ** 2).sum()
+    rhs = kstep ** 2 / np.pi * (N * (x ** 2).sum() + X0 ** 2 - XN ** 2) / 2
+    assert abs(lhs - rhs) < 1e-12 * rhs
+    model = data + 0.1 * rng.normal(size=361)
+    for win in (FitWindow(3, 14, 1, 2, 1, 6), FitWindow(2, 16, 2, 3, 1.2, 5.3), FitWindow(4, 12, 0, 1, 0.5, 9)):
+        m0, nm = r_range(win, kstep, N, float32_grid=False)
+        a = r_np(k, data, model, kmin=win.kmin, kmax=win.kmax, rmin=win.rmin, rmax=win.rmax, kweight=win.kweight,
+                 nfft=N, dk=win.dk)
+        b = oracle.r_factor(model, data, kstep, win.kmin, win.kmax, win.dk, win.kweight, m0, nm, N)[0]
+        assert abs(a - b) < 1e-12 * a
