@@ -296,6 +296,20 @@ def main():
     misfit = dict(kernel="k_r_factor", segments=int(nseg), r_points=rloss.nm, launch_ms=t_rf,
                   note="R-factor per frame and d R / d chi_seg in one launch (FP64)")
 
+    # single scattering of whole frames from pair tables: every atom of every frame an absorber, pairs within 5 A
+    pairs = None
+    if cell is not None:
+        from feffgpu_amd.pairs import PairTables
+        pt = PairTables(str(PHASE_K), q, box=cell, rmax=5.0, r_min=1.5, dr=0.002, ms=False, device=dev)
+        Pf = pos_t.reshape(nseg, -1, 3).clone().requires_grad_(True)
+        gk = torch.tensor(q ** 2, device=dev)
+        t_ss = launch_ms(lambda: pt.single_scattering(Pf.detach()))
+        t_ssb = launch_ms(lambda: torch.autograd.grad((pt.single_scattering(Pf) * gk).sum(), Pf))
+        npair = int(Pf.shape[0]) * int(Pf.shape[1]) ** 2
+        pairs = dict(kernels="k_pair_hist (forward), k_pair_grad (adjoint)", frames=int(Pf.shape[0]),
+                     atoms=int(Pf.shape[1]), pair_evaluations=npair, forward_ms=t_ss, forward_backward_ms=t_ssb,
+                     note="every atom an absorber, minimum image, rmax 5 A; forward includes the histogram-table product")
+
     out = dict(metric=METRIC, value=units / (ms_fb32 * 1e-3), unit="paths·k-pts/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=ms_fb32, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="fp32 (fp64 geometry and phase)", data="synthetic",
@@ -314,7 +328,7 @@ def main():
                parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
                                   backward_ms=policy["backward"]),
-               roofline=roof, roofline_forward=roof_fwd, misfit=misfit)
+               roofline=roof, roofline_forward=roof_fwd, misfit=misfit, pair_tables=pairs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, pl, q, cell, args.cpu_seconds)
     print(json.dumps(out), flush=True)

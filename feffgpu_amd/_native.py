@@ -51,6 +51,11 @@ class FmWindow(ctypes.Structure):
                 ("kweight", _i32), ("rmin", ctypes.c_double), ("rmax", ctypes.c_double)]
 
 
+class FmPairGrid(ctypes.Structure):
+    _fields_ = [("r0", ctypes.c_double), ("dr", ctypes.c_double), ("rmax", ctypes.c_double), ("nr", _i32),
+                ("cell_kind", _i32), ("cell", ctypes.c_double * 9), ("cellinv", ctypes.c_double * 9)]
+
+
 def grid_runs(q, jq_rows, E_list):
     """qorder (q indices in ascending q) and qstart [nedge][stride]: for every edge, the first position in qorder
     whose lower interpolation index is >= e, for e = 0..E (the runs the adjoint of the fine grid gathers)."""
@@ -108,6 +113,9 @@ def lib():
                                          ctypes.c_size_t, _vp]
         L.fm_r_factor.argtypes = [_i32, _i32, ctypes.c_double, _i32, ctypes.POINTER(FmWindow), _i32, _i32, _vp,
                                   ctypes.c_double, _vp, _i32, _vp, _vp, _vp]
+        L.fm_pair_histogram.argtypes = [_i32, _i32, _vp, _i32, _vp, _vp, _i32, ctypes.POINTER(FmPairGrid), _vp, _vp,
+                                        _vp]
+        L.fm_pair_histogram_grad.argtypes = [_i32, _i32, _vp, _vp, _i32, ctypes.POINTER(FmPairGrid), _vp, _vp, _vp]
         L.fm_set_lds_budget.restype = _i32
         L.fm_set_lds_budget.argtypes = [_i32, _i32]
         _lib = L

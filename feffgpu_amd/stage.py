@@ -132,7 +132,7 @@ class GPUPathStage:
     def spline_matrix(self, q):
         """Cubic-spline interpolation matrix (len(q), E) (scipy UnivariateSpline s=0, as Larch); q clamped to the
         grid. Host-built once, as in src/feffgpu/stage.py:341-351."""
-        S = tables_spline_matrix(self.P.xk[: self.E] / BOHR, q)
+        S = tables_spline_matrix(self.P.xk[: self.E] / BOHR, self._qarray(q))
         return torch.tensor(S, device=self.device, dtype=self.rdtype)
 
     def linear_matrix(self, q):
@@ -142,7 +142,7 @@ class GPUPathStage:
         S = self._linear.get(key)
         if S is None:
             kf = self.P.xk[: self.E] / BOHR
-            j, f = interpolation(kf, q)
+            j, f = interpolation(kf, self._qarray(q))
             m = np.zeros((len(j), self.E))
             m[np.arange(len(j)), j] = 1 - f
             m[np.arange(len(j)), j + 1] = f

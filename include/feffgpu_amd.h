@@ -195,6 +195,30 @@ int fm_r_factor(int32_t nseg, int32_t nk, double kstep, int32_t nfft, const fm_w
                 const double *chi, double scale, const double *data, int32_t data_stride, double *rfac, double *gchi,
                 void *stream);
 
+/* The distance grid and cell of single scattering from tables (PairTables, src/feffgpu/exafs/forward.py:178-320). */
+typedef struct fm_pair_grid {
+    double r0, dr;               /* grid r0 + i dr, i = 0 .. nr - 1 (A) */
+    double rmax;                 /* pairs within rmax (A) count */
+    int32_t nr;                  /* >= 2 */
+    int32_t cell_kind;           /* 0: cluster, 1: cubic box cell[0], 2: general cell (rows = lattice vectors) */
+    double cell[9], cellinv[9];  /* host values */
+} fm_pair_grid;
+
+/* PairTables.histogram (src/feffgpu/exafs/forward.py:233-280) for nconf configurations pos [nconf][natom][3] (A):
+ * every pair (absorber a, neighbor j != a) within rmax, minimum image, adds 1 - f to bin i0 and f to bin i0 + 1 of
+ * row ipot[j] (u = (r - r0) / dr, i0 = floor(u) clamped to 0..nr-2, f = u - i0 clamped to [0, 1]). absorbers
+ * [nabs] (indices into each configuration) or NULL for every atom; ipot [natom] in 0..npot-1 (other values: never a
+ * neighbor). hist [npot][nr] accumulates in fixed point, units of 2^-40 of a pair (the caller zeroes it; a bin holds
+ * up to 2^23 pairs): order-independent. nshort (or NULL) += the pairs closer than r0 (placed on the first bin). */
+int fm_pair_histogram(int32_t nconf, int32_t natom, const double *pos, int32_t nabs, const int32_t *absorbers,
+                      const int32_t *ipot, int32_t npot, const fm_pair_grid *grid, int64_t *hist, int32_t *nshort,
+                      void *stream);
+
+/* Adjoint of the histogram with every atom an absorber (PairTables.single_scattering's autograd, forward.py:284-320):
+ * gbin [npot][nr] = dL/dhist (FP64) -> grad_pos [nconf][natom][3] = dL/dpos, one thread per atom, no atomics. */
+int fm_pair_histogram_grad(int32_t nconf, int32_t natom, const double *pos, const int32_t *ipot, int32_t npot,
+                           const fm_pair_grid *grid, const double *gbin, double *grad_pos, void *stream);
+
 /* Launch policy (process-wide): the LDS in KiB (16..160) that a workgroup may take when the number of paths per
  * workgroup is chosen, for kernel 0 (FP32 forward calls), 1 (FP32 adjoints), 2 (FP64 forward), 3 (FP64 adjoints);
  * defaults 80 (two FP32 forward workgroups per CU), then 160. Set by feffgpu_amd.tune from timings. Returns the

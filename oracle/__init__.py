@@ -219,3 +219,42 @@ def r_factor(chi, data, kstep, kmin, kmax, dk, kweight, m0, nm, nfft=2048, scale
     M = chi_r(scale * np.atleast_2d(chi), kstep, kmin, kmax, dk, kweight, nfft)[:, m0:m0 + nm]
     D = chi_r(np.atleast_2d(data), kstep, kmin, kmax, dk, kweight, nfft)[:, m0:m0 + nm]
     return (np.abs(M - D) ** 2).sum(1) / (np.abs(D) ** 2).sum(1)
+
+
+# ------------------------------------------------------------------------------------------------ pair histograms
+def min_image(rel, cell):
+    """Minimum image of displacements rel (..., 3) (src/feffgpu/cell.py:26-35): cell None, a cubic length or 3 x 3
+    (rows the lattice vectors)."""
+    rel = np.asarray(rel, float)
+    if cell is None:
+        return rel
+    c = np.asarray(cell, float)
+    if c.ndim == 0:
+        return rel - float(c) * np.round(rel / float(c))
+    f = rel @ np.linalg.inv(c)
+    return (f - np.round(f)) @ c
+
+
+def pair_histogram(pos, absorbers, ipot, cell, r0, dr, nr, rmax, npot):
+    """PairTables.histogram (src/feffgpu/exafs/forward.py:233-280) in numpy: every pair (absorber a, neighbor j)
+    within rmax and not coincident adds 1 - f to bin i0 and f to bin i0 + 1 of row ipot[j]. Returns (hist
+    (npot, nr), pairs closer than r0)."""
+    pos = np.asarray(pos, float)
+    ipot = np.asarray(ipot)
+    hist = np.zeros((npot, nr))
+    short = 0
+    for a in np.asarray(absorbers).reshape(-1):
+        rel = min_image(pos - pos[a], cell)
+        r2 = (rel ** 2).sum(1)
+        m = (r2 <= rmax ** 2) & (r2 > 1e-6)
+        j = np.nonzero(m)[0]
+        r = np.sqrt(r2[m])
+        short += int((r < r0).sum())
+        u = (r - r0) / dr
+        i0 = np.clip(np.floor(u), 0, nr - 2).astype(int)
+        f = np.clip(u - i0, 0.0, 1.0)
+        p = ipot[j]
+        keep = (p >= 0) & (p < npot)
+        np.add.at(hist, (p[keep], i0[keep]), 1.0 - f[keep])
+        np.add.at(hist, (p[keep], i0[keep] + 1), f[keep])
+    return hist, short

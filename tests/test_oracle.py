@@ -251,3 +251,39 @@ def test_misfit_oracle_pins():
                  nfft=N, dk=win.dk)
         b = oracle.r_factor(model, data, kstep, win.kmin, win.kmax, win.dk, win.kweight, m0, nm, N)[0]
         assert abs(a - b) < 1e-12 * a
+
+
+def test_pair_histogram_oracle_pins():
+    """oracle.pair_histogram against what fixes it: every pair within the cutoff adds a mass of one; linear weights
+    keep the first moment (sum_i hist_i r_i = sum of the pair distances, for pairs on the grid); a perfect bcc
+    lattice puts its shells (8 at a sqrt(3)/2, 6 at a, 12 at a sqrt(2) per atom) on their bins; a cubic cell given as
+    3 x 3 gives the cubic result."""
+    from feffgpu_amd import workloads as W
+    a = 3.0
+    pos0, box = W.lattice("bcc", a, 4)
+    L = float(box[0])
+    n = len(pos0)
+    r0, dr, rmax = 1.5, 0.002, 4.5
+    nr = int(round((rmax - r0) / dr)) + 1
+    h, short = oracle.pair_histogram(pos0, np.arange(n), np.ones(n, int), L, r0, dr, nr, rmax, 2)
+    assert short == 0 and abs(h[0]).max() == 0.0
+    for r, cnt in ((a * np.sqrt(3) / 2, 8), (a, 6), (a * np.sqrt(2), 12)):
+        u = (r - r0) / dr
+        i0 = int(np.floor(u))
+        assert abs(h[1, i0] + h[1, i0 + 1] - cnt * n) < 1e-9 * n
+    pos = W.displaced(pos0, 0.08, 3)
+    rng = np.random.default_rng(2)
+    ipot = rng.integers(0, 3, n)
+    ab = rng.choice(n, 17, replace=False)
+    h, short = oracle.pair_histogram(pos, ab, ipot, L, r0, dr, nr, rmax, 3)
+    rel = np.concatenate([oracle.min_image(pos - pos[i], L) for i in ab])
+    nb = np.concatenate([ipot for _ in ab])
+    r = np.sqrt((rel ** 2).sum(1))
+    sel = (r > 1e-3) & (r <= rmax)
+    assert abs(h.sum() - sel.sum()) < 1e-9 * sel.sum()
+    grid = r0 + dr * np.arange(nr)
+    for p in range(3):
+        on = sel & (nb == p) & (r >= r0)
+        assert abs((h[p] * grid).sum() - r[on].sum()) < 1e-9 * max(r[on].sum(), 1.0)
+    h2, _ = oracle.pair_histogram(pos, ab, ipot, np.eye(3) * L, r0, dr, nr, rmax, 3)
+    assert np.abs(h2 - h).max() < 1e-12
