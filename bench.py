@@ -306,7 +306,7 @@ def main():
         t_ss = launch_ms(lambda: pt.single_scattering(Pf.detach()))
         t_ssb = launch_ms(lambda: torch.autograd.grad((pt.single_scattering(Pf) * gk).sum(), Pf))
         npair = int(Pf.shape[0]) * int(Pf.shape[1]) ** 2
-        pairs = dict(kernels="k_pair_hist (forward), k_pair_grad (adjoint)", frames=int(Pf.shape[0]),
+        pairs = dict(kernels="k_pair_hist_lds (forward), k_pair_grad (adjoint)", frames=int(Pf.shape[0]),
                      atoms=int(Pf.shape[1]), pair_evaluations=npair, forward_ms=t_ss, forward_backward_ms=t_ssb,
                      note="every atom an absorber, minimum image, rmax 5 A; forward includes the histogram-table product")
 

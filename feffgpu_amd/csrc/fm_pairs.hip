@@ -95,6 +95,58 @@ __global__ __launch_bounds__(PH_THREADS) void k_pair_hist(int natom, const doubl
     }
 }
 
+// The same histogram with the bins of PH_ABS absorbers first summed in LDS (64-bit fixed point, LDS atomics) and only
+// the non-zero bins then added to hist: the pairs of a frame crowd onto the few bins of its coordination shells, so
+// global atomics on them would contend; integer sums stay order-independent.
+constexpr int PH_ABS = 64;
+
+__global__ __launch_bounds__(PH_THREADS) void k_pair_hist_lds(int natom, const double *pos, int nabs, const int32_t *absorbers,
+                                                               const int32_t *ipot, int npot, fm_pair_grid g,
+                                                               unsigned long long *hist, int32_t *nshort) {
+    extern __shared__ __align__(16) unsigned long long lh[];
+    __shared__ int cnt[PH_THREADS / FM_WAVE];
+    const int c = blockIdx.y, t = threadIdx.x;
+    const int nb = npot * g.nr;
+    for (int b = t; b < nb; b += PH_THREADS) lh[b] = 0ull;
+    __syncthreads();
+    const double *P = pos + (size_t)c * natom * 3;
+    int shorts = 0;
+    const int a_end = min(nabs, (int)(blockIdx.x + 1) * PH_ABS);
+    for (int ai = blockIdx.x * PH_ABS; ai < a_end; ++ai) {
+        const int a = absorbers ? absorbers[ai] : ai;
+        if (a < 0 || a >= natom) continue;          // uniform over the workgroup
+        const double pa[3] = {P[3 * a], P[3 * a + 1], P[3 * a + 2]};
+        for (int j = t; j < natom; j += PH_THREADS) {
+            double v[3], r;
+            if (!pair(pa, P + 3 * (size_t)j, g, v, r)) continue;
+            if (r < g.r0) ++shorts;
+            const int p = ipot[j];
+            if (p < 0 || p >= npot) continue;
+            int i0;
+            double f, dfdr;
+            bin_of(r, g, i0, f, dfdr);
+            const unsigned long long up = (unsigned long long)llrint(f * PH_ONE);
+            atomicAdd(&lh[p * g.nr + i0], (unsigned long long)PH_ONE - up);
+            atomicAdd(&lh[p * g.nr + i0 + 1], up);
+        }
+    }
+    __syncthreads();
+    for (int b = t; b < nb; b += PH_THREADS) {
+        const unsigned long long v = lh[b];
+        if (v) atomicAdd(hist + b, v);
+    }
+    if (nshort) {
+        for (int off = FM_WAVE / 2; off > 0; off >>= 1) shorts += __shfl_xor(shorts, off, FM_WAVE);
+        if ((t & (FM_WAVE - 1)) == 0) cnt[t / FM_WAVE] = shorts;
+        __syncthreads();
+        if (t == 0) {
+            int sum = 0;
+            for (int w = 0; w < PH_THREADS / FM_WAVE; ++w) sum += cnt[w];
+            if (sum) atomicAdd(nshort, sum);
+        }
+    }
+}
+
 // d L / d pos for L = sum_p,i gbin[p][i] hist[p][i], every atom an absorber: pair (a, j) puts (1 - f, f) on the
 // bins (i0, i0 + 1) of potential ipot[j]; its distance moves with pos_j - pos_a.
 __global__ __launch_bounds__(PH_THREADS) void k_pair_grad(int natom, const double *pos, const int32_t *ipot, int npot,
@@ -162,8 +214,19 @@ int fm_pair_histogram(int32_t nconf, int32_t natom, const double *pos, int32_t n
     if (na < 0 || nconf > 65535) return fm_set_err(FM_EINVAL, "fm_pair_histogram: bad absorber count or more than 65535 configurations");
     if (nconf == 0 || natom == 0 || na == 0) return FM_OK;
     if (!pos || !ipot || !hist) return fm_set_err(FM_EINVAL, "fm_pair_histogram: NULL buffer");
-    hipLaunchKernelGGL(k_pair_hist, dim3((unsigned)na, (unsigned)nconf), dim3(PH_THREADS), 0, (hipStream_t)stream, natom, pos,
-                       na, absorbers, ipot, npot, *grid, reinterpret_cast<unsigned long long *>(hist), nshort);
+    const size_t lds = sizeof(unsigned long long) * (size_t)npot * grid->nr;
+    if (lds <= 144 * 1024) {   // the bins fit a workgroup's LDS: sum PH_ABS absorbers there first
+        if (lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute((const void *)k_pair_hist_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return fm_set_err(FM_EHIP, hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(k_pair_hist_lds, dim3((unsigned)((na + PH_ABS - 1) / PH_ABS), (unsigned)nconf), dim3(PH_THREADS),
+                           lds, (hipStream_t)stream, natom, pos, na, absorbers, ipot, npot, *grid,
+                           reinterpret_cast<unsigned long long *>(hist), nshort);
+    } else {
+        hipLaunchKernelGGL(k_pair_hist, dim3((unsigned)na, (unsigned)nconf), dim3(PH_THREADS), 0, (hipStream_t)stream, natom,
+                           pos, na, absorbers, ipot, npot, *grid, reinterpret_cast<unsigned long long *>(hist), nshort);
+    }
     return launch_err("fm_pair_histogram");
 }
 

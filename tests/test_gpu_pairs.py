@@ -125,3 +125,28 @@ def test_cell_must_exceed_twice_the_cutoff():
     from feffgpu_amd.pairs import PairTables
     with pytest.raises(ValueError):
         PairTables(str(PHASE), Q, box=10.0, rmax=5.6, ms=False)
+
+
+def test_histogram_without_lds_privatisation(tables):
+    """A grid too fine for a workgroup's LDS (3 x 8201 bins) takes the kernel with global atomics: same histogram."""
+    import ctypes
+    from feffgpu_amd import _native as N
+    pt, pos0, L = tables
+    pos = W.displaced(pos0, 0.1, 6)
+    n = len(pos)
+    g = pt._grid()
+    g.dr = 0.0005
+    g.nr = int(round((pt.rmax - pt.r0) / g.dr)) + 1
+    ipot = np.random.default_rng(5).integers(0, 3, n)
+    ab = np.arange(0, n, 3)
+    hist = torch.zeros(3, g.nr, dtype=torch.int64, device="cuda")
+    short = torch.zeros(1, dtype=torch.int32, device="cuda")
+    P = torch.tensor(pos, device="cuda")
+    abt = torch.tensor(ab, dtype=torch.int32, device="cuda")
+    ipt = torch.tensor(ipot, dtype=torch.int32, device="cuda")
+    N.check(N.lib().fm_pair_histogram(1, n, P.data_ptr(), len(ab), abt.data_ptr(), ipt.data_ptr(), 3, ctypes.byref(g),
+                                      hist.data_ptr(), short.data_ptr(), None), "fm_pair_histogram")
+    ref, rshort = oracle.pair_histogram(pos, ab, ipot, L, pt.r0, g.dr, g.nr, pt.rmax, 3)
+    err = float(np.abs(hist.cpu().numpy() / 2.0 ** 40 - ref).max())
+    print(f"[pairs] histogram of {g.nr} bins per potential: max abs {err:.1e}", flush=True)
+    assert err < 1e-9 and int(short.item()) == rshort
