@@ -587,7 +587,7 @@ __device__ __forceinline__ void bwd_row_c(const cx<T> (&K)[FM_NREP], const cx<T>
     cfma(dY, y1d, Q[0][1] * xa[0] + Q[1][1] * xa[1] + Q[2][1] * xa[2]);
 }
 
-// The tangent stream of one scattering matrix (rows il = 1..ilx, t-matrix one row ahead), complex derivatives.
+// The tangent stream of one scattering matrix (rows il = 1..ilx, t-matrix two rows ahead), complex derivatives.
 template <class T, int SET, int NR, int NC>
 __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
                                              const T *__restrict__ Wl, const T *__restrict__ Wdl,
@@ -598,26 +598,31 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
     X.init(minus_i_over(ck * rX), rX);
     Y.init(minus_i_over(ck * rY), rY);
     cx<T> xa[3], xd[3], ya[2], yd[2];
+    // the t-matrix two rows ahead (rows past ilx repeat row ilx, which stays in the table)
     cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
+    const size_t r2 = (size_t)(ilx < 2 ? ilx : 2) * tlstride;
+    cx<T> tn = mk<T>(tl[r2], tl[r2 + 1]);
     const int rseed = ilx < 4 ? ilx : 4;
     for (int r = 1; r <= rseed; ++r) {
         X.seed_row(r, xa, xd);
         if (r < 4) Y.seed_row(r, ya, yd); else Y.step(r, ya, yd);
-        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
-        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
+        const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
         bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
                                   Wdl + (size_t)r * FM_NREP, dX, dY, dB);
         t = tn;
+        tn = tnn;
     }
 #pragma unroll 1
     for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa, xd);
         Y.step(r, ya, yd);
-        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
-        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
+        const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
         bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
                                   Wdl + (size_t)r * FM_NREP, dX, dY, dB);
         t = tn;
+        tn = tnn;
     }
 }
 

@@ -309,7 +309,7 @@ __device__ __forceinline__ void accum_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&xa
 // F_s of the path at the lane's energy: X = Hankel factors of leg s+1 (gam), Y = of leg s (gamtl) times the
 // scatterer's t-matrix, summed over il <= ilx with the rotation table of leg s (src/feffgpu/stage.py:287-305).
 // Accumulates the representatives k with ri < NR, rj < NC into Ft (zeroed here). Each lane runs its own rows
-// il = 1..ilx (the wave runs to the largest ilx of its lanes); the t-matrix of the next row is fetched one row ahead.
+// il = 1..ilx (the wave runs to the largest ilx of its lanes); the t-matrix is fetched two rows ahead.
 template <class T, int SET, int NR, int NC>
 __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
                                        const T *__restrict__ Wl, const T *__restrict__ tl, int tlstride) {
@@ -320,24 +320,29 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
     X.init(minus_i_over(ck * rX));
     Y.init(minus_i_over(ck * rY));
     cx<T> xa[3], yb[2];
+    // the t-matrix two rows ahead (rows past ilx repeat row ilx, which stays in the table)
     cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
+    const size_t r2 = (size_t)(ilx < 2 ? ilx : 2) * tlstride;
+    cx<T> tn = mk<T>(tl[r2], tl[r2 + 1]);
     const int rseed = ilx < 4 ? ilx : 4;
     for (int r = 1; r <= rseed; ++r) {
         X.seed_row(r, xa);
         if (r < 4) Y.seed_row(r, yb); else Y.step(r, yb);
-        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
-        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
+        const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
         accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
         t = tn;
+        tn = tnn;
     }
 #pragma unroll 1
     for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa);
         Y.step(r, yb);
-        const size_t rn = (size_t)(r < ilx ? r + 1 : r) * tlstride;
-        const cx<T> tn = mk<T>(tl[rn], tl[rn + 1]);
+        const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
+        const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
         accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
         t = tn;
+        tn = tnn;
     }
 }
 
