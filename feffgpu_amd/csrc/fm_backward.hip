@@ -1307,7 +1307,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             const int p = a.order ? a.order[a.first + s0] : a.first + s0;
             PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)(act ? j : 0) * a.path_bytes);
             int ei;
-            load_path_group(a.src, p, g, ei, NL, lane & 15, act);
+            load_path_group(a.src, p, g, ei, NL, lane & 15, act, SET < FM_SET_SS);
         }
         if constexpr (SET < FM_SET_SS) build_tables<T, SET, true, 16>(ed, smem, a.path_bytes, a.off_W, a.off_Wd, wid, 4, G, NL, ltab);
         __syncthreads();
@@ -1355,7 +1355,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) void k_adjoint_tail(BArgs a) {
     double *tmp = gp + 3 * FM_MAX_LEGS;
     const int p = a.order ? a.order[a.first + s0] : a.first + s0;
     int ei;
-    load_path(a.src, p, g, ei, NL);
+    load_path(a.src, p, g, ei, NL, false);
     const cx<T> *wp = reinterpret_cast<const cx<T> *>(a.work) + (size_t)s0 * a.dstride;
     for (int e = lane; e < E; e += FM_WAVE) cc[e] = wp[e];
     wave_sync();

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             const int p = a.order ? a.order[a.first + s0] : a.first + s0;
             PathLDS &g = *reinterpret_cast<PathLDS *>(smem + (size_t)(act ? j : 0) * a.path_bytes);
             int ei;
-            load_path_group(a.src, p, g, ei, nl, lane & 15, act);
+            load_path_group(a.src, p, g, ei, nl, lane & 15, act, false);   // single scattering: no tables
             if (act && (lane & 15) == 0) { pidx[j] = p; pval[j] = valid ? 1 : 0; }
         } else {
             for (int j = wid; j < G; j += 4) {

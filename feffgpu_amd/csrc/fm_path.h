@@ -116,7 +116,8 @@ __device__ inline void path_point(const fm_paths &P, int p, int j, double (&v)[3
 // of a launch has nuni legs (the kernels loop over the legs uniformly): a path-list entry with another leg count is
 // replaced by a fixed placeholder path of nuni legs with g.ok = 0, so that every index stays in bounds; the kernels
 // write NaN for it.
-__device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni, int lane, bool act) {
+__device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni, int lane, bool act,
+                                      bool beta_trig = true) {
     int n;
     bool ok = true;
     if (a.mode == 0) {
@@ -171,11 +172,13 @@ __device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &
         g.c1[J] = c; g.s1[J] = s;
         sincos(2.0 * eta, &s, &c);
         g.c2[J] = c; g.s2[J] = s;
-        const double be = g.beta[J];
-        sincos(be, &s, &c);
-        g.cb[J] = c; g.sb[J] = s;
-        sincos(0.5 * be, &s, &c);
-        g.ch[J] = c; g.sh[J] = s;
+        if (beta_trig) {   // for the rotation tables only
+            const double be = g.beta[J];
+            sincos(be, &s, &c);
+            g.cb[J] = c; g.sb[J] = s;
+            sincos(0.5 * be, &s, &c);
+            g.ch[J] = c; g.sh[J] = s;
+        }
     }
     if (act && lane == 0) {
         double R = 0.0, pr = 1.0;
@@ -186,8 +189,8 @@ __device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &
     return n;
 }
 
-__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni) {
-    return load_path_group(a, p, g, edge_idx, nuni, lane_id(), true);
+__device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni, bool beta_trig = true) {
+    return load_path_group(a, p, g, edge_idx, nuni, lane_id(), true, beta_trig);
 }
 
 // The recursion coefficients of B consecutive rows il0.. of one (m1, m2) (mm, nn as build_tables forms them),
