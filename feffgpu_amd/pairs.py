@@ -85,6 +85,8 @@ class PairTables:
                     cchi, reff = stage.cchi(coords, ipots)
                     rows.append(stage.chi_fine(cchi, reff, self.q, self.S, "feff").float())
                 self.tables[ipot] = torch.cat(rows)
+        if not self.tables:
+            raise ValueError("the phase file has no scatterer potential")
         self.C, self.n = 0, 0
 
     # -- the distance grid and the cell as the kernels take them
@@ -117,7 +119,9 @@ class PairTables:
         g = self._grid(box)
         hist = torch.zeros(self.npot, self.nr, dtype=torch.int64, device=self.device)
         short = torch.zeros(1, dtype=torch.int32, device=self.device)
-        N.check(N.lib().fm_pair_histogram(1, n, P.data_ptr(), ab.numel(), ab.data_ptr() if ab.numel() else None,
+        if ab.numel() == 0 or n == 0:        # no absorber: an empty histogram (a NULL list would mean every atom)
+            return hist.double(), 0
+        N.check(N.lib().fm_pair_histogram(1, n, P.data_ptr(), ab.numel(), ab.data_ptr(),
                                           ip.data_ptr(), self.npot, ctypes.byref(g), hist.data_ptr(), short.data_ptr(),
                                           N.stream_of(self.device)), "fm_pair_histogram")
         return hist.double() / _ONE, int(short.item())
@@ -159,6 +163,8 @@ class _SingleScattering(torch.autograd.Function):
         if not (torch.is_tensor(pos) and pos.is_cuda):
             raise RuntimeError("single_scattering runs on a ROCm device: pos must be a device tensor")
         C, n, _ = pos.shape
+        if C * n == 0:
+            raise ValueError("single_scattering needs at least one atom in one configuration")
         P = pos.detach().to(torch.float64).contiguous()
         ip = (torch.as_tensor(pt.ipot, device=pos.device) if hasattr(pt, "ipot")
               else torch.ones(n, dtype=torch.int64, device=pos.device)).to(torch.int32).contiguous()

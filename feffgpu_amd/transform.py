@@ -95,8 +95,10 @@ def r_factor_raw(chi, data, window: FitWindow, kstep=KSTEP, nfft=NFFT, scale=1.0
     Returns rfac (S,) and, with grad, d rfac / d chi (S, NK)."""
     chi = chi.contiguous()
     data = data.contiguous()
+    if chi.dim() != 2:
+        raise ValueError("chi must be (S, NK)")
     S, nk = chi.shape
-    if data.dim() == 1 and data.shape[0] != nk or data.dim() == 2 and tuple(data.shape) != (S, nk):
+    if not ((data.dim() == 1 and data.shape[0] == nk) or (data.dim() == 2 and tuple(data.shape) == (S, nk))):
         raise ValueError("data must be (NK,) or (S, NK) like chi")
     m0, nm = r_range(window, kstep, nfft, float32_grid)
     w = N.FmWindow(float(window.kmin), float(window.kmax), float(window.dk), int(window.kweight), float(window.rmin),
@@ -146,8 +148,8 @@ class RFactorLoss:
         self.kstep, nk = _kstep_of(q)
         self.window, self.scale, self.nfft = window, float(scale), int(nfft)
         self.data = torch.as_tensor(data, dtype=torch.float64, device=device).contiguous()
-        if self.data.shape[-1] != nk:
-            raise ValueError("data must have one value per point of q")
+        if self.data.dim() not in (1, 2) or self.data.shape[-1] != nk:
+            raise ValueError("data must be (NK,) or (S, NK), one value per point of q")
         self.m0, self.nm = r_range(window, self.kstep, self.nfft, float32_grid=False)
         self._w = N.FmWindow(float(window.kmin), float(window.kmax), float(window.dk), int(window.kweight),
                             float(window.rmin), float(window.rmax))
@@ -156,6 +158,8 @@ class RFactorLoss:
 
     def __call__(self, seg):
         S, nk = seg.shape
+        if self.data.dim() == 2 and self.data.shape[0] != S:
+            raise ValueError("data holds another number of segments than chi_seg")
         if self.rfac is None or self.rfac.shape[0] != S:
             self.rfac = torch.empty(S, dtype=torch.float64, device=seg.device)
             self._g = torch.empty(S, nk, dtype=torch.float64, device=seg.device)

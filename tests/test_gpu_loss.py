@@ -137,3 +137,16 @@ def test_captured_step_with_r_factor_loss():
     an = float((g_eager.cpu().numpy() * d).sum())
     print(f"[loss] directional derivative of the R-factor sum: adjoint {an:.10e}, FD {fd:.10e}", flush=True)
     assert abs(an - fd) < 1e-6 * np.abs(g_eager.cpu().numpy()).sum() + 1e-8 * abs(fd)
+
+
+def test_shape_checks():
+    from feffgpu_amd.transform import FitWindow, RFactorLoss, r_factor_raw
+    x = torch.zeros(2, 361, dtype=torch.float64, device="cuda")
+    for bad in (torch.zeros(3, 361, dtype=torch.float64, device="cuda"), torch.zeros(360, dtype=torch.float64, device="cuda"),
+                torch.zeros(1, 2, 361, dtype=torch.float64, device="cuda")):
+        with pytest.raises(ValueError):
+            r_factor_raw(x, bad, FitWindow())
+    q = np.arange(0, 18.0001, 0.05)
+    loss = RFactorLoss(np.zeros((3, len(q))), q, FitWindow())
+    with pytest.raises(ValueError):
+        loss(x)

@@ -150,3 +150,14 @@ def test_histogram_without_lds_privatisation(tables):
     err = float(np.abs(hist.cpu().numpy() / 2.0 ** 40 - ref).max())
     print(f"[pairs] histogram of {g.nr} bins per potential: max abs {err:.1e}", flush=True)
     assert err < 1e-9 and int(short.item()) == rshort
+
+
+def test_edge_cases(tables):
+    """No absorber gives an empty histogram (not every atom's); an empty configuration is refused."""
+    pt, pos0, L = tables
+    n = len(pos0)
+    P = torch.tensor(pos0, device="cuda")
+    h, short = pt.histogram(P, torch.zeros(0, dtype=torch.int64, device="cuda"), torch.ones(n, dtype=torch.int64, device="cuda"))
+    assert float(h.abs().sum()) == 0.0 and short == 0
+    with pytest.raises(ValueError):
+        pt.single_scattering(torch.zeros(1, 0, 3, device="cuda", dtype=torch.float64))
