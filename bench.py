@@ -167,7 +167,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print("bench.py measures one GPU: this build runs the path stage on a single MI355X (no multi-GPU path)",
+        print("bench.py measures one GPU: the path stage runs on a single MI355X (no multi-GPU path)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -273,8 +273,8 @@ def main():
     t_bwd = launch_ms(lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
     t_fwd = launch_ms(lambda: plan32.forward_bucket(pos_t, bdom))
     peak = MI355X_PEAK_TFLOPS["fp32"]
-    note = ("FP32 vector pipes (no MFMA); 157.3 is MI355X's FP32 peak with packed math, which this build disables "
-            "(non-packed VALU ceiling 78.6)")
+    note = ("FP32 vector pipes (no MFMA); 157.3 is MI355X's FP32 peak with packed math, which these kernels are "
+            "compiled without (non-packed VALU ceiling 78.6)")
     ach_b = 2 * b_work[bdom] / (t_bwd * 1e-3) / 1e12
     ach_f = b_work[bdom] / (t_fwd * 1e-3) / 1e12
     nl = int(plan32.buckets.nleg[bdom])

@@ -1,6 +1,6 @@
 """The adjoint kernels against finite differences of the FP64 oracle and against each other.
 
-Gradient bar (north star): relative error <= 1e-4 of the FP32 adjoint against the FP64 reference gradient. The FP64
+Gradient bar: relative error <= 1e-4 of the FP32 adjoint against the FP64 reference gradient. The FP64
 adjoint is first pinned against five-point central differences of the FP64 oracle (h = 1e-4 A, generic geometry: no
 collinear legs, where FEFF's regularised angles are not differentiable), to 1e-6 of the largest component.
 """

@@ -2,7 +2,7 @@
 
 Tolerances: FP64 kernels 1e-9 of max|cchi| per path (the reference's own stage-vs-port bound is 1e-8,
 tests/test_pathfinder.py:229); FP32 kernels (FP64 geometry and phase) 2e-5 of max|cchi| per path and, for the
-summed chi, R-factor <= 2e-5 and max|dchi|/max|chi| <= 1e-5 (the north-star parity bar).
+summed chi, R-factor <= 2e-5 and max|dchi|/max|chi| <= 1e-5 (the parity bar of the FP32 path).
 """
 import numpy as np
 import pytest
