@@ -1547,7 +1547,7 @@ constexpr size_t FM_DREGION_MAX = (size_t(1) << 31) - 4096;   // the sweep addre
 
 // The fused adjoint (multiple scattering of 3..FM_FUSED_MAXLEG legs, single scattering): k_adjoint's sweep, then
 // k_adjoint_tail, over chunks of the bucket of as many paths as the workspace holds regions for (one spare). Single
-// scattering keeps little in registers: two workgroups per CU (FP32).
+// scattering of K edges keeps little in registers: two workgroups per CU (FP32).
 template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
     const int nl = SET >= FM_SET_SS ? 2 : a.nlegmax;
