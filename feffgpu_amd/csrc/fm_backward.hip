@@ -986,8 +986,11 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
         if (c2 > 0.0) v = mk<T>((T)(ab * cr / a - pb * ci / c2), (T)(ab * ci / a + pb * cr / c2));
         cb[e] = v;
     };
+    // staging: dL/dA and dL/dPhi per q point (values already rounded to T, so stored as T), then the fractions
+    T *const sA = qs ? reinterpret_cast<T *>(qs) : nullptr;
+    T *const sP = qs ? sA + nk : nullptr;
+    double *const sF = qs ? reinterpret_cast<double *>(sP + nk) : nullptr;
     if (S) {   // dense interpolation (qs required): dL/dA, dL/dPhi per q point, then S^T per energy
-        double *sA = qs, *sP = qs + nk;
         for (int i = lane; i < nk; i += FM_WAVE) {
             const double *Si = S + (size_t)i * sstride;
             double sa = 0.0, sp = 0.0;
@@ -1002,8 +1005,8 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
             T sn, cs;
             if constexpr (sizeof(T) == 4) sincosf((float)ph, &sn, &cs); else sincos(ph, &sn, &cs);
             const double gP = (double)(T)(gu * sa * (double)cs);
-            sA[i] = (double)(T)(gu * (double)sn);
-            sP[i] = gP;
+            sA[i] = (T)(gu * (double)sn);
+            sP[i] = (T)gP;
             rb += 2.0 * q[i] * gP;
         }
         wave_sync();
@@ -1011,22 +1014,21 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
             double ab = 0.0, pb = 0.0;
             for (int i = 0; i < nk; ++i) {
                 const double s = S[(size_t)i * sstride + e];
-                ab += s * sA[i];
-                pb += s * sP[i];
+                ab += s * (double)sA[i];
+                pb += s * (double)sP[i];
             }
             finish(e, ab, pb);
         }
         return wave_sum(rb);
     }
     if (qs) {
-        double *sA = qs, *sP = qs + nk, *sF = qs + 2 * nk;
         const int t1 = qstart[E];
         for (int t = lane; t < t1; t += FM_WAVE) {
             const int i = qorder[t];
             double gA, gP;
             adj(i, gA, gP);
-            sA[t] = gA;
-            sP[t] = gP;
+            sA[t] = (T)gA;
+            sP[t] = (T)gP;
             sF[t] = fq[i];
             rb += 2.0 * q[i] * gP;
         }
@@ -1035,12 +1037,12 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
             double ab = 0.0, pb = 0.0;
             const int lo0 = qstart[e], lo1 = qstart[e + 1], hi0 = e > 0 ? qstart[e - 1] : lo0;
             for (int t = lo0; t < lo1; ++t) {
-                ab += (1.0 - sF[t]) * sA[t];
-                pb += (1.0 - sF[t]) * sP[t];
+                ab += (1.0 - sF[t]) * (double)sA[t];
+                pb += (1.0 - sF[t]) * (double)sP[t];
             }
             for (int t = hi0; t < lo0; ++t) {
-                ab += sF[t] * sA[t];
-                pb += sF[t] * sP[t];
+                ab += sF[t] * (double)sA[t];
+                pb += sF[t] * (double)sP[t];
             }
             finish(e, ab, pb);
         }
@@ -1528,7 +1530,7 @@ template <class T> __host__ void layout_tail(BArgs &a, const fm_edge &ed) {
     a.off_red = o; o += align16(sizeof(double) * (nacc(a.nlegmax) + 6 * FM_MAX_LEGS));
     const size_t qs = sizeof(double) * 3 * (size_t)a.nk;
     a.off_qs = 0;
-    if (a.src.mode == 1 && qs > 0 && qs <= FM_QS_MAX) { a.off_qs = o; o += align16(qs); }
+    if (a.src.mode == 1 && qs > 0 && qs <= FM_QS_MAX) { a.off_qs = o; o += align16((2 * sizeof(T) + sizeof(double)) * (size_t)a.nk); }
     a.off_gs = o; o += align16(sizeof(double) * 3 * (size_t)a.nlegmax * a.nlegmax);
     a.path_bytes = o;
 }
