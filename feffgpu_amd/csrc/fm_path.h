@@ -595,7 +595,7 @@ __device__ void fine_grid(const fm_edge &ed, const int32_t *jq, const double *fq
 // chi_seg[s][k] = sum over the CSR list of segment s of deg[p] chi_path[p][k]: a workgroup per (64 k-points,
 // segment); wave w sums the list entries w, w + 8, ... in order, then the 8 partial sums are added in wave order.
 // Deterministic for a given list and grid.
-constexpr int SEG_WAVES = 8;
+constexpr int SEG_WAVES = 16;
 template <class T>
 __global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, int nk, const int32_t *seg_ptr,
                                                                      const int32_t *seg_paths, const double *deg,
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, in
     double acc = 0.0;
     if (k < nk) {
         const int t0 = seg_ptr[s] + w, t1 = seg_ptr[s + 1];
-        constexpr int U = 8;   // paths whose loads are in flight together; summed in list order
+        constexpr int U = 16;   // paths whose loads are in flight together; summed in list order
         for (int tb = t0; tb < t1; tb += U * SEG_WAVES) {
             int pv[U];
 #pragma unroll
