@@ -28,10 +28,7 @@ def hipcc() -> str:
     raise FileNotFoundError("hipcc not found (ROCm)")
 
 
-def build(verbose: bool = False) -> Path:
-    srcs = [str(HERE / s) for s in SOURCES if (HERE / s).exists()]
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-o", str(tmp), *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -40,6 +37,21 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{err}")
     if verbose and err.strip():
         print(err)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compiles every source to an object in parallel (one hipcc per file), then links the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    srcs = [HERE / s for s in SOURCES if (HERE / s).exists()]
+    objdir = HERE / "_build"
+    objdir.mkdir(exist_ok=True)
+    objs = [objdir / (s.stem + ".o") for s in srcs]
+    cflags = [f for f in FLAGS if f != "-shared"]
+    jobs = jobs or min(len(srcs), os.cpu_count() or 1)
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([hipcc(), *cflags, "-c", "-o", str(so[1]), str(so[0])], verbose), zip(srcs, objs)))
+    tmp = OUT.with_suffix(".so.tmp")
+    _run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)], verbose)
     tmp.replace(OUT)
     return OUT
 

@@ -1683,6 +1683,7 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     a.src.nleg = nleg;
     a.src.coords = coords;
     a.src.ipots64 = ipots;
+    a.src.nph1 = edge->nph1;
     a.gcchi = gcchi;
     a.greff = greff;
     a.gcoord = gcoords;

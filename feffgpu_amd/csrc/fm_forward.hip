@@ -124,8 +124,8 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             if (active) {
                 if (a.src.mode == 0) {
                     T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)pidx[jl] * E + e) * 2;
-                    o[0] = c.r;
-                    o[1] = c.i;
+                    o[0] = gl.ok ? c.r : (T)__builtin_nan("");
+                    o[1] = gl.ok ? c.i : (T)__builtin_nan("");
                 } else {
                     T *amp = reinterpret_cast<T *>(lbase + off_amp);
                     double *ang = reinterpret_cast<double *>(lbase + off_ang);
@@ -261,6 +261,7 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
     a.src.nleg = nleg;
     a.src.coords = coords;
     a.src.ipots64 = ipots;
+    a.src.nph1 = edge->nph1;
     a.cchi_out = cchi;
     a.reff_out = reff;
     a.edge_id = 0;

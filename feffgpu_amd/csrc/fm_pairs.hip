@@ -2,8 +2,9 @@
 // of absorber-neighbor distances on the tables' grid r0 + i dr (linear-interpolation weights, minimum image of the
 // cell, pairs within rmax, self excluded) and its adjoint in the positions.
 //
-// The histogram is accumulated in 64-bit fixed point (units of 2^-40 of a pair): integer atomics are associative, so
-// the result does not depend on the order the pairs arrive in, and each pair adds exactly 2^40 over its two bins. The
+// The histogram is accumulated in 64-bit fixed point (units of 2^-32 of a pair): integer atomics are associative, so
+// the result does not depend on the order the pairs arrive in, and each pair adds exactly 2^32 over its two bins. A
+// bin holds up to 2^31 pairs before the signed 64-bit value overflows (the host splits larger calls). The
 // adjoint needs no atomics: one thread per atom sums its pairs, as absorber and as neighbor, in index order.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -16,7 +17,7 @@ int fm_set_err(int code, const char *msg);
 namespace {
 
 constexpr int PH_THREADS = 256;
-constexpr double PH_ONE = 1099511627776.0;   // 2^40
+constexpr double PH_ONE = 4294967296.0;   // 2^32
 
 // minimum image of v (src/feffgpu/cell.py:26-35): cubic v - L rint(v / L), general cell by fractional rounding
 __device__ __forceinline__ void min_image(double (&v)[3], const fm_pair_grid &g) {
@@ -214,6 +215,10 @@ int fm_pair_histogram(int32_t nconf, int32_t natom, const double *pos, int32_t n
     if (na < 0 || nconf > 65535) return fm_set_err(FM_EINVAL, "fm_pair_histogram: bad absorber count or more than 65535 configurations");
     if (nconf == 0 || natom == 0 || na == 0) return FM_OK;
     if (!pos || !ipot || !hist) return fm_set_err(FM_EINVAL, "fm_pair_histogram: NULL buffer");
+    // every ordered pair adds one unit over two bins: nconf * na * (natom - 1) < 2^31 keeps every bin below overflow
+    if ((double)nconf * na * (natom - 1) >= 2147483648.0)
+        return fm_set_err(FM_EINVAL, "fm_pair_histogram: nconf * absorbers * (natom - 1) >= 2^31 pairs could overflow a "
+                                     "bin; split the configurations over several calls");
     const size_t lds = sizeof(unsigned long long) * (size_t)npot * grid->nr;
     if (lds <= 144 * 1024) {   // the bins fit a workgroup's LDS: sum PH_ABS absorbers there first
         if (lds > 64 * 1024) {

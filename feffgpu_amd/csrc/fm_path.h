@@ -22,6 +22,7 @@ struct PathSrc {
     int nleg;                 // mode 0
     const double *coords;     // mode 0: [npath][nleg][3] A
     const int64_t *ipots64;   // mode 0: [npath][nleg]
+    int nph1;                 // mode 0: potentials 0..nph1-1 exist; a path naming another is a placeholder (NaN)
     fm_paths P;               // mode 1
 };
 
@@ -115,7 +116,8 @@ __device__ inline void path_point(const fm_paths &P, int p, int j, double (&v)[3
 // this path as lanes ln = 0..15 of a group (several groups of a wavefront fill several paths at once). Every path
 // of a launch has nuni legs (the kernels loop over the legs uniformly): a path-list entry with another leg count is
 // replaced by a fixed placeholder path of nuni legs with g.ok = 0, so that every index stays in bounds; the kernels
-// write NaN for it.
+// write NaN for it. In mode 0 a path that names a potential outside 0..nph1-1 reads the tables of potential 0 and
+// has g.ok = 0 (NaN output), so the caller need not check its potentials on the host.
 __device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &edge_idx, int nuni, int lane, bool act,
                                       bool beta_trig = true) {
     int n;
@@ -128,8 +130,14 @@ __device__ inline int load_path_group(const PathSrc &a, int p, PathLDS &g, int &
             double x = c[0] / FM_BOHR, y = c[1] / FM_BOHR, z = c[2] / FM_BOHR;
             // absorber (lane 0) is the path's first point; the stage places it at rat[0] and rat[n]
             g.rat[lane][0] = x; g.rat[lane][1] = y; g.rat[lane][2] = z;
-            g.ipot[lane] = (int)a.ipots64[(size_t)p * n + lane];
+            const int64_t ip = a.ipots64[(size_t)p * n + lane];
+            g.ipot[lane] = (ip >= 0 && ip < a.nph1) ? (int)ip : 0;    // tables are read at potential 0 instead
         }
+        if (act && lane == 0)
+            for (int j = 0; j < n; ++j) {
+                const int64_t ip = a.ipots64[(size_t)p * n + j];
+                if (ip < 0 || ip >= a.nph1) ok = false;
+            }
     } else {
         ok = a.P.nleg[p] == nuni;
         n = nuni;

@@ -19,7 +19,8 @@ from . import _native as N
 from .stage import GPUPathStage
 from .transform import ETOK
 
-_ONE = float(2 ** 40)
+_ONE = float(2 ** 32)        # the histogram's fixed-point unit (2^-32 of a pair)
+_BIN_PAIRS = 2 ** 31          # pairs one call may add (a bin holds fewer than 2^31 pairs)
 
 
 def _cell(box):
@@ -169,10 +170,17 @@ class _SingleScattering(torch.autograd.Function):
         ip = (torch.as_tensor(pt.ipot, device=pos.device) if hasattr(pt, "ipot")
               else torch.ones(n, dtype=torch.int64, device=pos.device)).to(torch.int32).contiguous()
         g = pt._grid()
-        hist = torch.zeros(pt.npot, pt.nr, dtype=torch.int64, device=pos.device)
-        N.check(N.lib().fm_pair_histogram(C, n, P.data_ptr(), 0, None, ip.data_ptr(), pt.npot, ctypes.byref(g),
-                                          hist.data_ptr(), None, N.stream_of(pos.device)), "fm_pair_histogram")
-        h = hist.double() / _ONE
+        # configurations in chunks of at most _BIN_PAIRS ordered pairs (the fixed-point bins' capacity), summed in FP64
+        per = max(1, (_BIN_PAIRS - 1) // max(1, n * (n - 1)))
+        h = torch.zeros(pt.npot, pt.nr, dtype=torch.float64, device=pos.device)
+        hist = torch.empty(pt.npot, pt.nr, dtype=torch.int64, device=pos.device)
+        for c0 in range(0, C, per):
+            cc = min(per, C - c0)
+            hist.zero_()
+            N.check(N.lib().fm_pair_histogram(cc, n, P[c0:c0 + cc].data_ptr(), 0, None, ip.data_ptr(), pt.npot,
+                                              ctypes.byref(g), hist.data_ptr(), None, N.stream_of(pos.device)),
+                    "fm_pair_histogram")
+            h += hist.double() / _ONE
         pots = sorted(pt.tables)
         T = pt.table_stack(pots)                                   # (P, nr, NK) float64
         chi = sum(torch.mv(T[i].t().contiguous(), h[p]) for i, p in enumerate(pots)) / (C * n)

@@ -34,10 +34,11 @@ def _require_gpu(device: torch.device):
 class _Cchi(torch.autograd.Function):
     @staticmethod
     def forward(ctx, coords, ipots, stage):
-        out, reff = stage._cchi_raw(coords, ipots)
+        out, reff64 = stage._cchi_raw(coords, ipots)
+        stage._reff64_last = reff64
         ctx.stage = stage
         ctx.save_for_backward(coords, ipots)
-        return out, reff
+        return out, reff64.to(stage.rdtype)
 
     @staticmethod
     def backward(ctx, gcchi, greff):
@@ -48,17 +49,18 @@ class _Cchi(torch.autograd.Function):
 
 class _ChiFine(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cchi, reff, stage, grid):
-        chi = stage._chi_fine_raw(cchi, reff, grid)
-        ctx.stage, ctx.grid = stage, grid
-        ctx.save_for_backward(cchi, reff)
+    def forward(ctx, cchi, reff, stage, grid, reff64):
+        r64 = reff.detach().to(torch.float64) if reff64 is None else reff64
+        chi = stage._chi_fine_raw(cchi, r64, grid)
+        ctx.stage, ctx.grid, ctx.rdtype = stage, grid, reff.dtype
+        ctx.save_for_backward(cchi, r64)
         return chi
 
     @staticmethod
     def backward(ctx, gchi):
-        cchi, reff = ctx.saved_tensors
-        gc, gr = ctx.stage._chi_fine_backward_raw(cchi, reff, ctx.grid, gchi)
-        return gc, gr, None, None
+        cchi, r64 = ctx.saved_tensors
+        gc, gr = ctx.stage._chi_fine_backward_raw(cchi, r64, ctx.grid, gchi)
+        return gc, gr.to(ctx.rdtype), None, None, None
 
 
 class GPUPathStage:
@@ -95,23 +97,37 @@ class GPUPathStage:
         coords = torch.as_tensor(coords)
         if coords.dim() != 3 or coords.shape[-1] != 3:
             raise ValueError("coords must be (B, nleg, 3)")
-        ipots = torch.as_tensor(ipots, device=self.device)
+        ipots = torch.as_tensor(ipots)
         if tuple(ipots.shape) != tuple(coords.shape[:2]):
             raise ValueError("ipots must be (B, nleg)")
-        return _Cchi.apply(coords, ipots, self)
+        if ipots.device.type == "cpu" and ipots.numel() and (int(ipots.min()) < 0 or int(ipots.max()) >= self.tables.nph1):
+            raise ValueError(f"potential index out of range 0..{self.tables.nph1 - 1}")
+        out, reff = _Cchi.apply(coords, ipots.to(self.device), self)
+        # The FP64 effective length behind the returned reff: chi_fine uses it when it is given this very tensor
+        # (unchanged), so that 2 q reff (hundreds of radians) carries no FP32 rounding in complex64 mode.
+        reff._fm_reff64 = (self._reff64_last, reff._version)
+        self._reff64_last = None
+        return out, reff
+
+    def _reff64_of(self, reff):
+        """The FP64 values behind a reff tensor returned by cchi, or None (another tensor, or changed since)."""
+        tag = getattr(reff, "_fm_reff64", None) if torch.is_tensor(reff) else None
+        if tag is None or tag[1] != reff._version or tag[0].shape != reff.shape:
+            return None
+        return tag[0]
 
     def _cchi_raw(self, coords, ipots):
+        """Potentials outside 0..nph of a path on the device are not checked here (that would synchronise the
+        device): the kernel reads potential 0's tables instead and writes NaN for that path."""
         B, nleg, _ = coords.shape
         c = coords.detach().to(device=self.device, dtype=torch.float64).contiguous()
         ip = ipots.to(device=self.device, dtype=torch.int64).contiguous()
-        if B and (int(ip.min()) < 0 or int(ip.max()) >= self.tables.nph1):
-            raise ValueError(f"potential index out of range 0..{self.tables.nph1 - 1}")
         out = torch.empty(B, self.E, dtype=self.cdtype, device=self.device)
         reff = torch.empty(B, dtype=torch.float64, device=self.device)
         if B:
             N.check(N.lib().fm_cchi(self._edge.struct, self.precision, self.iorder, B, nleg, c.data_ptr(), ip.data_ptr(),
                                     out.data_ptr(), reff.data_ptr(), N.stream_of(self.device)), "fm_cchi")
-        return out, reff.to(self.rdtype)
+        return out, reff
 
     def _cchi_backward_raw(self, coords, ipots, gcchi, greff):
         B, nleg, _ = coords.shape
@@ -198,9 +214,11 @@ class GPUPathStage:
         cubic spline_matrix(q) for "larch", or the S given. Returns (B, len(q)) in the stage's real dtype;
         differentiable in cchi and reff. The matrix linear_matrix returned runs as the kernels' own linear
         interpolation; any other S as a dense interpolation matrix."""
+        reff = torch.as_tensor(reff, device=self.device)
+        r64 = self._reff64_of(reff)
         if S is None:
             if convention == "feff":
-                return _ChiFine.apply(cchi, torch.as_tensor(reff, device=self.device), self, self._grid(q))
+                return _ChiFine.apply(cchi, reff, self, self._grid(q), r64)
             if convention != "larch":
                 raise ValueError("convention must be 'feff' or 'larch'")
             key = self._qkey(q)
@@ -210,7 +228,7 @@ class GPUPathStage:
                 S = self._spline[key] = torch.tensor(tables_spline_matrix(kf, self._qarray(q)), device=self.device)
         elif S is self._linear.get(self._qkey(q)):
             S = None
-        return _ChiFine.apply(cchi, torch.as_tensor(reff, device=self.device), self, self._grid(q, S))
+        return _ChiFine.apply(cchi, reff, self, self._grid(q, S), r64)
 
     def _chi_fine_raw(self, cchi, reff, grid):
         c = cchi.detach().to(self.cdtype).contiguous()
@@ -233,4 +251,4 @@ class GPUPathStage:
             N.check(N.lib().fm_chi_fine_backward(self._edge.struct, self.precision, B, c.data_ptr(), r.data_ptr(),
                                                  grid[0], g.data_ptr(), gc.data_ptr(), gr.data_ptr(),
                                                  N.stream_of(self.device)), "fm_chi_fine_backward")
-        return gc, gr.to(reff.dtype)
+        return gc, gr

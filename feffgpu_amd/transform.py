@@ -27,8 +27,11 @@ RSTEP = math.pi / (KSTEP * NFFT)   # A
 
 @dataclass(frozen=True)
 class FitWindow:
-    """kmin, kmax: centers of the Hanning sills (1/A); dk: their width; kweight: power of k; rmin, rmax: the R range
-    of the R-factor (A). Same fields, defaults and checks as src/feffgpu/exafs/transform.py:30-62."""
+    """Where a model is compared with a spectrum. The field names and defaults are the reference's interface
+    (src/feffgpu/exafs/transform.py:28-62), and so are the conditions that raise ValueError.
+
+    The Hanning window rises over a sill of width dk centred on kmin and falls over one centred on kmax (1/A); chi is
+    multiplied by k**kweight before the transform; the R-factor sums over rmin <= R <= rmax (A)."""
     kmin: float = 3.0
     kmax: float = 14.0
     dk: float = 1.0
@@ -38,18 +41,19 @@ class FitWindow:
 
     def __post_init__(self):
         if not (self.kmax > self.kmin >= 0.0):
-            raise ValueError(f"the k window needs 0 <= kmin < kmax (got kmin {self.kmin}, kmax {self.kmax})")
+            raise ValueError(f"FitWindow: kmin={self.kmin}, kmax={self.kmax}; kmin must be >= 0 and below kmax")
         if not (self.rmax > self.rmin >= 0.0):
-            raise ValueError(f"the R window needs 0 <= rmin < rmax (got rmin {self.rmin}, rmax {self.rmax})")
+            raise ValueError(f"FitWindow: rmin={self.rmin}, rmax={self.rmax}; rmin must be >= 0 and below rmax")
         if self.dk < 0:
-            raise ValueError(f"dk must not be negative (got {self.dk})")
+            raise ValueError(f"FitWindow: sill width dk={self.dk} is negative")
 
     def as_dict(self):
         return asdict(self)
 
     @property
     def independent_points(self):
-        """2 (kmax - kmin) (rmax - rmin) / pi + 2 (xraylarch's nidp)."""
+        """The number of independent points of the window, nidp = 2 dk_range dr_range / pi + 2 (Stern's estimate, as
+        xraylarch counts it)."""
         return 2.0 * (self.kmax - self.kmin) * (self.rmax - self.rmin) / math.pi + 2.0
 
 
@@ -131,7 +135,9 @@ def r_factor_torch(model, data, k, window: FitWindow):
     normalisation and R grid are KSTEP's and NFFT's, the R range is taken on the float32 R grid."""
     if not (torch.is_tensor(model) and model.is_cuda):
         raise RuntimeError("r_factor_torch runs on a ROCm device: model must be a device tensor")
-    _kstep_of(k)
+    step, _ = _kstep_of(k)
+    if abs(step - KSTEP) > 1e-9:   # the kernel builds k_j = j KSTEP for k^w and the window
+        raise ValueError(f"k must be the uniform grid from zero with step {KSTEP} (got step {step})")
     m = model.double()
     d = torch.as_tensor(data, device=model.device).double()
     return _RFactor.apply(m, d, window, KSTEP)

@@ -208,8 +208,9 @@ typedef struct fm_pair_grid {
  * every pair (absorber a, neighbor j != a) within rmax, minimum image, adds 1 - f to bin i0 and f to bin i0 + 1 of
  * row ipot[j] (u = (r - r0) / dr, i0 = floor(u) clamped to 0..nr-2, f = u - i0 clamped to [0, 1]). absorbers
  * [nabs] (indices into each configuration) or NULL for every atom; ipot [natom] in 0..npot-1 (other values: never a
- * neighbor). hist [npot][nr] accumulates in fixed point, units of 2^-40 of a pair (the caller zeroes it; a bin holds
- * up to 2^23 pairs): order-independent. nshort (or NULL) += the pairs closer than r0 (placed on the first bin). */
+ * neighbor). hist [npot][nr] accumulates in fixed point, units of 2^-32 of a pair (the caller zeroes it):
+ * order-independent. A bin holds up to 2^31 pairs, so a call with nconf * absorbers * (natom - 1) >= 2^31 is refused
+ * (FM_EINVAL) and the caller splits its configurations. nshort (or NULL) += the pairs closer than r0 (placed on the first bin). */
 int fm_pair_histogram(int32_t nconf, int32_t natom, const double *pos, int32_t nabs, const int32_t *absorbers,
                       const int32_t *ipot, int32_t npot, const fm_pair_grid *grid, int64_t *hist, int32_t *nshort,
                       void *stream);

@@ -79,17 +79,16 @@ def test_cchi_adjoint_matches_fd_and_fp32(case, nlegs):
 
 
 def test_autograd_through_cchi_and_chi_fine():
-    """Autograd through the drop-in API (cchi -> chi_fine -> loss). In complex64 mode cchi returns reff in float32,
-    as the reference does; 2 q reff reaches ~200 rad, so that rounding alone puts the FP32 gradient of a chi-level
-    loss at ~1e-4 of its largest component (with reff kept in FP64 it is <= 2e-5); the fused entry, which keeps reff
-    in FP64 internally, meets 1e-4 by a wide margin (test_chi_paths_gradient_fd_and_precisions)."""
+    """Autograd through the drop-in API (cchi -> chi_fine -> loss) against five-point differences of the FP64 oracle.
+    In complex64 mode cchi returns reff in float32, as the reference does, and chi_fine evaluates 2 q reff (hundreds
+    of radians) with the FP64 value behind that tensor, so the FP32 gradient meets the 1e-4 bar."""
     from feffgpu_amd.stage import GPUPathStage
     phase, paths = _paths("validation/A018", (3, 4), 4, 3)
     orc = oracle.OraclePhase(phase)
     q = np.arange(0.0, 18.0001, 0.05)
     rng = np.random.default_rng(4)
     wq = rng.normal(size=len(q)) * q ** 2
-    for dtype, tol in ((torch.complex128, 1e-6), (torch.complex64, 5e-4)):
+    for dtype, tol in ((torch.complex128, 1e-6), (torch.complex64, 1e-4)):
         st = GPUPathStage(str(phase), dtype=dtype)
         for coords, ipots in paths:
             ct = torch.tensor(coords[None], device="cuda", dtype=torch.float64, requires_grad=True)
@@ -109,7 +108,9 @@ def test_autograd_through_cchi_and_chi_fine():
                     e[i, k] = 1.0
                     gfd[i, k] = _d(L, coords, e)
             err = np.abs(g - gfd).max() / np.abs(gfd).max()
+            print(f"[grad] drop-in autograd {dtype}: {len(ipots)} legs, max rel {err:.2e}", flush=True)
             assert err < tol, (dtype, err)
+            assert reff.dtype == (torch.float32 if dtype == torch.complex64 else torch.float64)
 
 
 def _box(seed, absorbers, rmax=4.6, npot=2):

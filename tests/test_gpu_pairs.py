@@ -38,14 +38,14 @@ def test_histogram_matches_oracle(tables):
         err = float(np.abs(h.cpu().numpy() - ref).max())
         print(f"[pairs] histogram cell {None if cell is None else np.shape(cell)}: {ref.sum():.0f} pairs, max abs {err:.1e}, "
               f"short {short}/{rshort}", flush=True)
-        assert err < 1e-9 and short == rshort
+        assert err < 1e-8 and short == rshort          # 2^-32 fixed point: <= 2^-33 per pair and bin
     # pairs closer than the grid's start land on its first bin and are counted
     close = pos.copy()
     close[1] = close[0] + np.array([0.0, 0.0, 1.2])
     h, short = pt.histogram(torch.tensor(close, device="cuda"), torch.tensor([0, 1], device="cuda"),
                             torch.ones(n, dtype=torch.int64, device="cuda"), box=L)
     ref, rshort = oracle.pair_histogram(close, [0, 1], np.ones(n, int), L, pt.r0, pt.dr, pt.nr, pt.rmax, pt.npot)
-    assert short == rshort >= 2 and float(np.abs(h.cpu().numpy() - ref).max()) < 1e-9
+    assert short == rshort >= 2 and float(np.abs(h.cpu().numpy() - ref).max()) < 1e-8
     with pytest.raises(ValueError):
         pt.histogram(torch.tensor(pos, device="cuda"), torch.tensor([n], device="cuda"), torch.tensor(ipot, device="cuda"))
 
@@ -149,7 +149,7 @@ def test_histogram_without_lds_privatisation(tables):
     ref, rshort = oracle.pair_histogram(pos, ab, ipot, L, pt.r0, g.dr, g.nr, pt.rmax, 3)
     err = float(np.abs(hist.cpu().numpy() / 2.0 ** 40 - ref).max())
     print(f"[pairs] histogram of {g.nr} bins per potential: max abs {err:.1e}", flush=True)
-    assert err < 1e-9 and int(short.item()) == rshort
+    assert err < 1e-8 and int(short.item()) == rshort
 
 
 def test_edge_cases(tables):

@@ -287,3 +287,126 @@ def test_pair_histogram_oracle_pins():
         assert abs((h[p] * grid).sum() - r[on].sum()) < 1e-9 * max(r[on].sum(), 1.0)
     h2, _ = oracle.pair_histogram(pos, ab, ipot, np.eye(3) * L, r0, dr, nr, rmax, 3)
     assert np.abs(h2 - h).max() < 1e-12
+
+
+def test_oracle_debye_waller_closed_form():
+    """The oracle's Debye-Waller factor (not in the reference; SURVEY's a19 form chi_p(q) exp(-2 sigma_p^2 q^2)):
+    per path of a list with a different sigma^2 for every path, the oracle's chi with sigma^2 equals its chi without
+    times the closed form evaluated here in numpy, on a grid with negative q (an e0 shift), and the segment sums are
+    the degeneracy-weighted sums of those rows. Also through the per-path fine grid (chi_fine)."""
+    from feffgpu_amd import workloads as W
+    phase = GOLDEN / "validation" / "A018" / "phase.pad"
+    ph = oracle.OraclePhase(phase)
+    pos0, box = W.lattice("bcc", 2.87, 3)
+    pos = W.displaced(pos0, 0.05, 4)
+    pl = W.concat([W.enumerate_paths(pos, a, np.full(len(pos), 2), 4.0, 4, box=box, segment=i, nmax=4)
+                   for i, a in enumerate([0, 9])], 4)
+    k = np.arange(0.0, 18.0001, 0.05)
+    en = k ** 2 - 5.0 * 0.2624684236075176                    # e0 = 5 eV: q < 0 below k = 1.15
+    q = np.sign(en) * np.sqrt(np.abs(en))
+    s2 = np.random.default_rng(1).uniform(0.0, 0.012, len(pl))
+    seg0, path0, _ = oracle.chi_paths(ph, pos, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, q,
+                                      segment=pl.segment, nseg=2, cell=box[0], want_paths=True)
+    seg, path, _ = oracle.chi_paths(ph, pos, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, q,
+                                    segment=pl.segment, nseg=2, sigma2=s2, cell=box[0], want_paths=True)
+    want = path0 * np.exp(-2.0 * s2[:, None] * q[None, :] ** 2)
+    assert np.abs(path - want).max() <= 1e-13 * np.abs(path0).max()
+    want_seg = np.zeros_like(seg)
+    np.add.at(want_seg, pl.segment, pl.deg[:, None] * want)
+    assert np.abs(seg - want_seg).max() <= 1e-12 * np.abs(want_seg).max()
+    # the per-path fine grid
+    from fixtures import load_case
+    _, paths, _, _ = load_case("validation/A018")
+    p = paths[3]
+    cc, reff, _ = ph.cchi(p.coords, p.ipots)
+    a = ph.chi_fine(cc, reff, q)[0]
+    b = ph.chi_fine(cc, reff, q, sigma2=0.0071)[0]
+    assert np.abs(b - a * np.exp(-2 * 0.0071 * q ** 2)).max() <= 1e-14 * np.abs(a).max()
+
+
+def test_oracle_debye_waller_is_the_gaussian_average():
+    """Physics pin of the same factor: a single-scattering path averaged over Gaussian bond lengths r ~ N(R, s^2)
+    (24-point Gauss-Hermite quadrature of the oracle's sigma^2 = 0 chi) is chi(R) exp(-2 s^2 k^2), up to the
+    amplitude's dependence on r (1/r^2, mean free path), which adds a relative ~2 k s^2 |d ln A / dr|. With
+    s = 0.03 A and k^2 weighting over k 4..16 the closed form is within 1.1% of the average; the plausible
+    mistakes (exp(-s^2 k^2), exp(-3 s^2 k^2), exp(-4 s^2 k^2), exp(+2 s^2 k^2), no factor) miss by 7% or more."""
+    phase = GOLDEN / "validation" / "A018" / "phase.pad"
+    ph = oracle.OraclePhase(phase)
+    R, s = 2.62, 0.03
+    k = np.arange(0.0, 18.0001, 0.05)
+    x, w = np.polynomial.hermite_e.hermegauss(24)
+    w = w / w.sum()
+
+    def chi_at(r, sigma2=None):
+        cc, reff, _ = ph.cchi(np.array([[0.0, 0.0, 0.0], [0.3 * r, -0.4 * r, np.sqrt(0.75) * r]]), np.array([0, 2]))
+        return ph.chi_fine(cc, reff, k, sigma2=sigma2)[0]
+    avg = sum(wi * chi_at(R + s * xi) for xi, wi in zip(x, w))
+    m = (k >= 4.0) & (k <= 16.0)
+    kw = k[m] ** 2
+
+    def miss(y):
+        return np.abs(kw * (avg[m] - y[m])).max() / np.abs(kw * avg[m]).max()
+    assert miss(chi_at(R, sigma2=s * s)) < 0.025
+    base = chi_at(R)
+    for wrong in (1.0, 3.0, 4.0, -2.0, 0.0):
+        assert miss(base * np.exp(-wrong * s * s * k ** 2)) > 0.05, wrong
+
+
+def _refgrad():
+    return np.load(GOLDEN / "refgrad" / "ref_autograd.npz")
+
+
+def test_oracle_matches_reference_vectors_at_arbitrary_geometry():
+    """Values written by the reference's own stage in complex128 (tests/golden/make_ref_autograd.py): exactly
+    collinear and axis-aligned paths, FEFF's W list, and the 2-8-leg random walks of the reference's
+    tests/test_pathfinder.py:193-213. The oracle's cchi within 1e-8 of the largest below k = 20 (the reference's own
+    bound against its literal port, tests/test_pathfinder.py:229; both keep every partial wave there), its reff to
+    1e-12 and its chi on the fine grid to 1e-8."""
+    d = _refgrad()
+    ph = oracle.OraclePhase(GOLDEN / "W" / "phase.pad")
+    lm, xk, ck = ph.tables()
+    low = xk / BOHR < 20.0
+    q = d["q"]
+    for g in d["groups"]:
+        C, I = d[f"coords_{g}"], d[f"ipots_{g}"]
+        for b in range(len(C)):
+            cc, reff, _ = ph.cchi(C[b], I[b])
+            ref = d[f"cchi_{g}"][b]
+            assert np.abs(cc - ref)[low].max() <= 1e-8 * np.abs(ref[low]).max(), (g, b)
+            assert abs(reff - d[f"reff_{g}"][b]) <= 1e-12 * reff
+            chi = ph.chi_fine(cc, reff, q)[0]
+            assert np.abs(chi - d[f"chi_{g}"][b]).max() <= 1e-8 * np.abs(d[f"chi_{g}"][b]).max(), (g, b)
+
+
+def test_reference_autograd_vectors_against_oracle_differences():
+    """Where the reference's autograd gradient of the cchi-level loss is finite, it equals five-point differences of
+    the oracle (1e-5 of the path's largest component: at exactly collinear legs the reference's regularised
+    beta = atan2(sqrt(|u1 x u2|^2 + 1e-30), b) leaves its smallest components a few 1e-7 off the differences); the
+    fixture's non-finite rows are the reference's own
+    (legs parallel to z: _trig's torch.where passes the 0/0 of d sqrt(x^2 + y^2) on, src/feffgpu/stage.py:147-157)
+    and are counted here so that a regenerated fixture that changes them is noticed."""
+    d = _refgrad()
+    ph = oracle.OraclePhase(GOLDEN / "W" / "phase.pad")
+    wc, wr = d["wc"], float(d["wr"])
+    h = 1e-4
+
+    def L(c, ip):
+        cc, r, _ = ph.cchi(c, ip)
+        return float((np.conj(wc) * cc).real.sum() + wr * r)
+    nonfinite = 0
+    for g in d["groups"]:
+        C, I, G = d[f"coords_{g}"], d[f"ipots_{g}"], d[f"gc_{g}"]
+        for b in range(0, len(C), 3):
+            if not np.isfinite(G[b]).all():
+                nonfinite += 1
+                continue
+            fd = np.zeros_like(C[b])
+            for i in range(C.shape[1]):
+                for k in range(3):
+                    e = np.zeros_like(C[b])
+                    e[i, k] = h
+                    fd[i, k] = (-L(C[b] + 2 * e, I[b]) + 8 * L(C[b] + e, I[b]) - 8 * L(C[b] - e, I[b])
+                                + L(C[b] - 2 * e, I[b])) / (12 * h)
+            assert np.abs(fd - G[b]).max() <= 1e-5 * np.abs(fd).max(), (g, b)
+    total = sum(int((~np.isfinite(d[f"gc_{g}"]).all((1, 2))).sum()) for g in d["groups"])
+    assert total == 33 and nonfinite > 0
