@@ -9,6 +9,7 @@ import torch
 
 import oracle
 from feffgpu_amd import workloads as W
+from feffgpu_amd.pairs import _ONE
 from fixtures import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -147,7 +148,7 @@ def test_histogram_without_lds_privatisation(tables):
     N.check(N.lib().fm_pair_histogram(1, n, P.data_ptr(), len(ab), abt.data_ptr(), ipt.data_ptr(), 3, ctypes.byref(g),
                                       hist.data_ptr(), short.data_ptr(), None), "fm_pair_histogram")
     ref, rshort = oracle.pair_histogram(pos, ab, ipot, L, pt.r0, g.dr, g.nr, pt.rmax, 3)
-    err = float(np.abs(hist.cpu().numpy() / 2.0 ** 40 - ref).max())
+    err = float(np.abs(hist.cpu().numpy() / _ONE - ref).max())
     print(f"[pairs] histogram of {g.nr} bins per potential: max abs {err:.1e}", flush=True)
     assert err < 1e-8 and int(short.item()) == rshort
 

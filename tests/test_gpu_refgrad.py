@@ -60,7 +60,9 @@ def test_stage_against_reference_autograd(prec):
             e_c = np.abs(cc[b] - rc[b]).max() / np.abs(rc[b]).max()
             e_x = np.abs(chi[b] - rchi[b]).max() / np.abs(rchi[b]).max()
             worst["cchi"], worst["chi"] = max(worst["cchi"], e_c), max(worst["chi"], e_x)
-            assert e_c <= tv and e_x <= tv * (1 if prec == 64 else 0.5), (g, b, e_c, e_x)
+            # FP32 cchi: 2e-5 of the path's largest value, 5e-5 for 7-8 legs (rounding over that many F products)
+            tc = tv if prec == 64 or C.shape[1] <= 6 else 5e-5
+            assert e_c <= tc and e_x <= tv * (1 if prec == 64 else 0.5), (g, b, e_c, e_x)
             assert abs(reff[b] - d[f"reff_{g}"][b]) <= (1e-12 if prec == 64 else 1e-6) * reff[b]
             assert np.isfinite(gc[b]).all() and np.isfinite(gq[b]).all(), (g, b)
             for name, mine, ref in (("gc", gc[b], d[f"gc_{g}"][b]), ("gq", gq[b], d[f"gq_{g}"][b])):
