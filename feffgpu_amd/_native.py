@@ -93,6 +93,8 @@ def lib():
         L.fm_cchi.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp]
         L.fm_cchi_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp,
                                        ctypes.c_size_t, _vp]
+        L.fm_cchi_backward_noalloc.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
+                                               _vp, _vp]
         L.fm_backward_workspace_bytes.restype = ctypes.c_size_t
         L.fm_backward_workspace_bytes.argtypes = [_i32, _i32, _i32]
         L.fm_chi_fine.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _vp, _vp, ctypes.POINTER(FmGrid), _vp, _vp]

@@ -20,12 +20,19 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "fm_device.h"
 #include "fm_path.h"
 
 using namespace fm;
+
+// The workspace of fm_cchi_backward_noalloc: a static device array, one per GPU the library runs on, held until the
+// process ends. Host code never passes `scratch` itself to a kernel (in host code the name is a stand-in, not a
+// device address): it looks the address up per device with hipGetSymbolAddress (scratch_of_current_device).
+constexpr size_t FM_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[FM_SCRATCH_BYTES];
 
 int fm_set_err(int code, const char *msg);
 size_t fm_lds_budget(int kernel);
@@ -1647,6 +1654,28 @@ int set_of(const fm_edge &ed, int nleg) {
     return ed.ilinit <= 1 ? 0 : 1;
 }
 
+constexpr int FM_SCRATCH_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[FM_SCRATCH_MAX_DEV];
+
+// The device address of the current device's `scratch`, looked up on the first call on that device.
+int scratch_of_current_device(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_err(e, "hipGetDevice");
+    if (dev < 0 || dev >= FM_SCRATCH_MAX_DEV) return set_err(FM_EINVAL, "fm_cchi_backward_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_err(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return set_err(FM_EHIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1697,6 +1726,28 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_cchi_backward: a path's tables do not fit one workgroup's LDS");
     if (e == hipErrorOutOfMemory) return set_err(FM_EINVAL, "fm_cchi_backward: workspace too small for one path");
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi_backward launch");
+}
+
+// fm_cchi_backward with the static device array `scratch` (FM_SCRATCH_BYTES per GPU) as its workspace: no allocation
+// by the library or the caller. Nothing in the workspace needs to start at zero (every slot the adjoint reads was
+// written earlier in the same call). Calls that overlap on one GPU share its array and race.
+int fm_cchi_backward_noalloc(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg,
+                             const double *coords, const int64_t *ipots, const void *gcchi, const double *greff,
+                             double *gcoords, void *stream) {
+    int rc = check_edge_b(edge);
+    if (rc) return rc;
+    if (nleg < 2 || nleg > FM_MAX_LEGS) return set_err(FM_EINVAL, "nleg out of range 2..9");
+    if (precision != 32 && precision != 64) return set_err(FM_EINVAL, "precision must be 32 or 64");
+    if (npath <= 0) return npath == 0 ? FM_OK : set_err(FM_EINVAL, "npath < 0");
+    // the least the adjoint works with: 16 band slots (7-9 legs) or one path's region and the spare (2-6 legs)
+    const size_t per_path = (size_t)(4 * (nleg < 6 ? nleg : 6) + 1) * edge->E * (precision == 32 ? 8 : 16);
+    if (fm_backward_workspace_bytes(precision, nleg, 16) > FM_SCRATCH_BYTES || 2 * per_path > FM_SCRATCH_BYTES)
+        return set_err(FM_EINVAL, "fm_cchi_backward_noalloc: one path of this edge needs more than the 64 MiB scratch");
+    void *ws = nullptr;
+    rc = scratch_of_current_device(&ws);
+    if (rc) return rc;
+    return fm_cchi_backward(edge, precision, iorder, npath, nleg, coords, ipots, gcchi, greff, gcoords, ws,
+                            FM_SCRATCH_BYTES, stream);
 }
 
 int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, const void *cchi, const double *reff,

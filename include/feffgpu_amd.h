@@ -111,6 +111,17 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
                      const double *coords, const int64_t *ipots, const void *gcchi, const double *greff, double *gcoords,
                      void *workspace, size_t workspace_bytes, void *stream);
 
+/* fm_cchi_backward without a workspace argument and without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Its workspace is a static device array of FM_NOALLOC_SCRATCH_BYTES (64 MiB) per GPU, held from
+ * the library's load to the end of the process; larger batches run in chunks of it, and an edge whose single path
+ * needs more than it (none of the shipped phase files) is refused with FM_EINVAL. Calls that overlap on one GPU share
+ * that array and race: serialise them (one stream per GPU). Same arguments and results as fm_cchi_backward. */
+#define FM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int fm_cchi_backward_noalloc(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npath, int32_t nleg,
+                             const double *coords, const int64_t *ipots, const void *gcchi, const double *greff,
+                             double *gcoords, void *stream);
+
 /* Bytes of the workspace the adjoints need for `band_slots` energy bands of paths of at most nlegmax legs in flight
  * at once (paths of 7-9 legs: each lane keeps its scattering matrices, prefix products and gradient terms there; a
  * workgroup holds one slot per band of its plan, at most 16, and a launch uses as many workgroups as the workspace
