@@ -12,7 +12,11 @@ A step is one pass of the path stage over the whole batch on the GPU with inputs
 d loss / d positions. Reported beside it: the fused FP64 kernel on the same workload, the roofline of the dominant
 kernel (HIP events on the launch stream), and the FP64 oracle timed on the host cores on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload md|fe] [--cpu-seconds 12]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload md|fe|cu|refine|multiedge] [--cpu-seconds 12]
+
+The other workloads are the remaining configurations: "cu" the fcc Cu cluster (k 0-16, sigma2 0.003), "fe" the bcc
+cluster, "refine" the 4000-atom NiO box (every 4th Ni an absorber), "multiedge" the 8000-atom Pt30Ni70 packing (L3
+and K edges, 2-6 legs). Each prints its own line with its roofline and its parity against the oracle.
 """
 from __future__ import annotations
 
@@ -29,10 +33,22 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "paths·k-pts/s fwd+bwd (FP32 fused vs FP64 plain, 1 GPU) at R-factor≤2e-5"
-PHASE_K = ROOT / "tests" / "golden" / "validation" / "A018" / "phase.pad"
+GOLD = ROOT / "tests" / "golden"
+PHASE_K = GOLD / "validation" / "A018" / "phase.pad"
+PHASE_L3 = GOLD / "validation" / "A014" / "phase.pad"
+PHASE_W = GOLD / "W" / "phase.pad"
 
 
-def build_workload(name: str):
+class Workload:
+    """A configuration of the benchmark: positions, the path list with its edges, the output grid, the cell, the
+    segments (frames or absorbers) and what the configuration is called."""
+
+    def __init__(self, pos, pl, q, cell, nseg, cfg, phases, edge=None, sigma2=None):
+        self.pos, self.pl, self.q, self.cell, self.nseg, self.cfg = pos, pl, q, cell, nseg, cfg
+        self.phases, self.edge, self.sigma2 = phases, edge, sigma2
+
+
+def build_workload(name: str) -> Workload:
     from feffgpu_amd import workloads as W
     q = np.round(np.arange(0, 18.0 + 1e-9, 0.05), 10)
     if name == "md":
@@ -51,14 +67,59 @@ def build_workload(name: str):
         cfg = dict(workload="md-batch bcc 2000 atoms x 64 frames, 16 absorbers/frame, nleg 2-4, Rmax 5 A, k 0-18/0.05",
                    atoms=n, frames=frames, absorbers_per_frame=nabs, rmax_A=5.0, nleg_max=4, nk=len(q),
                    phase="FEFF validation A018 V K edge (substitute for Fe K)", cell_A=float(box[0]))
-        return np.concatenate(pos_all), pl, q, float(box[0]), frames, cfg
+        return Workload(np.concatenate(pos_all), pl, q, float(box[0]), frames, cfg, [PHASE_K])
     if name == "fe":
         cl = W.cluster("bcc", 2.87, 9.0)
         pos = W.displaced(cl, 0.05, 7, keep_first=True)
         pl = W.enumerate_paths(pos, 0, np.full(len(pos), 2), 6.0, 4, nmax=4)
         cfg = dict(workload="bcc cluster 9 A (259 atoms), sigma 0.05 A, nleg <= 4, Rmax 6 A, k 0-18/0.05",
                    atoms=len(pos), rmax_A=6.0, nleg_max=4, nk=len(q), phase="FEFF validation A018 V K edge")
-        return pos, pl, q, None, 1, cfg
+        return Workload(pos, pl, q, None, 1, cfg, [PHASE_K])
+    if name == "cu":
+        cl = W.cluster("fcc", 3.61, 6.0)
+        pos = W.displaced(cl, 0.03, 101)
+        pos = pos - pos[0]
+        qc = np.round(np.arange(0, 16.0 + 1e-9, 0.05), 10)
+        pl = W.enumerate_paths(pos, 0, np.ones(len(pos), int), 5.0, 4, nmax=4)
+        cfg = dict(workload="fcc Cu cluster 6 A (79 atoms), sigma 0.03 A, nleg <= 4, Rmax 5 A, k 0-16/0.05, "
+                            "sigma2 0.003", atoms=len(pos), rmax_A=5.0, nleg_max=4, nk=len(qc),
+                   phase="FEFF W K edge (substitute for Cu K)")
+        return Workload(pos, pl, qc, None, 1, cfg, [PHASE_W], sigma2=0.003)
+    if name == "refine":
+        pos0, species, box = W.rocksalt(4.18, (10, 10, 5))
+        ipot = np.where(species == 0, 2, 1)
+        pos = W.displaced(pos0, 0.1, 41)
+        absorbers = np.nonzero(species == 0)[0][::4]
+        pl = W.concat([W.enumerate_paths(pos, int(a), ipot, 5.0, 4, box=box, segment=i, nmax=4)
+                       for i, a in enumerate(absorbers)], 4)
+        cfg = dict(workload=f"rocksalt NiO 4000 atoms (10x10x5 cells, a 4.18 A), sigma 0.1 A, {len(absorbers)} Ni "
+                            "absorbers (every 4th), nleg <= 4, Rmax 5 A, k 0-18/0.05, k^2 L2 loss",
+                   atoms=len(pos), absorbers=len(absorbers), rmax_A=5.0, nleg_max=4, nk=len(q),
+                   phase="FEFF validation A018 V K edge in Ge-V rock salt (substitute for Ni K in NiO)")
+        return Workload(pos, pl, q, np.diag(box), len(absorbers), cfg, [PHASE_K])
+    if name == "multiedge":
+        n = 8000
+        box = (n * (0.3 * 195.08 + 0.7 * 58.69) / 6.02214076e23 / 8.9 * 1e24) ** (1 / 3)
+        pos = W.random_packing(n, box, 2.3, 11)
+        heavy = np.zeros(n, bool)
+        heavy[np.random.default_rng(12).choice(n, 2400, replace=False)] = True
+        rng = np.random.default_rng(13)
+        pt_abs = rng.choice(np.nonzero(heavy)[0], 32, replace=False)
+        ni_abs = rng.choice(np.nonzero(~heavy)[0], 32, replace=False)
+        ipot_l3 = np.ones(n, int)                       # A014 has one scatterer potential
+        ipot_k = np.where(heavy, 1, 2)                  # A018: 1 for the heavy species, 2 for the light one
+        lists, edges = [], []
+        for e, (absr, ipot) in enumerate(((pt_abs, ipot_l3), (ni_abs, ipot_k))):
+            for a in absr:
+                lst = W.enumerate_paths(pos, int(a), ipot, 7.0, 6, box=np.full(3, box), segment=len(lists), nmax=6)
+                lists.append(lst)
+                edges.append(np.full(len(lst), e))
+        pl = W.concat(lists, 6)
+        cfg = dict(workload="random-packed Pt30Ni70 8000 atoms (8.9 g/cm3, box 52.98 A, d_min 2.3 A), Pt L3 + Ni K, "
+                            "32 absorbers per edge, nleg <= 6, Rmax 7 A, k 0-18/0.05", atoms=n, rmax_A=7.0,
+                   nleg_max=6, nk=len(q), cell_A=float(box),
+                   phase="FEFF validation A014 Zn L3 (substitute for Pt L3) and A018 V K (substitute for Ni K)")
+        return Workload(pos, pl, q, float(box), len(lists), cfg, [PHASE_L3, PHASE_K], edge=np.concatenate(edges))
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -91,38 +152,102 @@ def timed_replays(torch, step, steps, warmup, stream):
     return wall, float(np.median(per)), per
 
 
-def cpu_baseline(pos, pl, q, cell, seconds):
-    """The FP64 oracle (OpenMP over paths) on a bounded sample of the same path list."""
-    import oracle
-    ph = oracle.OraclePhase(PHASE_K)
-    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    threads = max(1, min(threads, os.cpu_count() or threads))
-    rng = np.random.default_rng(5)
+def _oracle_args(wl, idx):
+    pl = wl.pl
+    s2 = None if wl.sigma2 is None else np.broadcast_to(np.asarray(wl.sigma2, float), (len(pl),))[idx]
+    return dict(atoms=pl.atoms[idx], nleg=pl.nleg[idx], absorber=pl.absorber[idx], ipots=pl.ipots[idx],
+                deg=pl.deg[idx], segment=pl.segment[idx], nseg=wl.nseg, sigma2=s2,
+                edge=None if wl.edge is None else wl.edge[idx], cell=wl.cell)
 
-    def run(idx):
+
+def cpu_baseline(wl, seconds, threads):
+    """The FP64 oracle (OpenMP over paths) on a bounded sample of the workload's path list: its rate, and its
+    per-path chi of the sample (returned for the parity check)."""
+    import oracle
+    phs = [oracle.OraclePhase(p) for p in wl.phases]
+    rng = np.random.default_rng(5)
+    pl, q = wl.pl, wl.q
+
+    def run(idx, want=False):
+        a = _oracle_args(wl, idx)
         t0 = time.perf_counter()
-        oracle.chi_paths(ph, pos, pl.atoms[idx], pl.nleg[idx], pl.absorber[idx], pl.ipots[idx], pl.deg[idx], q,
-                         cell=cell, threads=threads)
-        return time.perf_counter() - t0
+        seg, path, _ = oracle.chi_paths(phs, wl.pos, a["atoms"], a["nleg"], a["absorber"], a["ipots"], a["deg"], q,
+                                        segment=a["segment"], nseg=a["nseg"], edge=a["edge"], sigma2=a["sigma2"],
+                                        cell=a["cell"], threads=threads, want_paths=want)
+        return time.perf_counter() - t0, seg, path
 
     probe = np.sort(rng.choice(len(pl), min(len(pl), 256 * threads), replace=False))
-    t = run(probe)
+    t, _, _ = run(probe)
     n = int(min(len(pl), max(len(probe), len(probe) * seconds / max(t, 1e-3))))
     idx = np.sort(rng.choice(len(pl), n, replace=False))
-    t = run(idx)
-    return dict(value=n * len(q) / t, unit="paths·k-pts/s", cores=threads, kind="port",
+    t, seg, path = run(idx, want=True)
+    base = dict(value=n * len(q) / t, unit="paths·k-pts/s", cores=threads, kind="port",
                 sample=f"{n} of {len(pl)} paths drawn at random (seed 5), all {len(q)} k-points, FP64 oracle "
                        f"(oracle/feffref.c, OpenMP), {t:.1f} s")
+    return base, idx, seg, path
+
+
+def oracle_parity(torch, wl, plan32, idx, seg_ref, path_ref, rmax, dev):
+    """The FP32 fused kernel against the FP64 oracle on the sampled paths: per-path chi (max|dchi_p| over the
+    sample relative to the sample's largest |chi_p|), the per-segment sums of the sample (R-factor, max|dchi|/max|chi|), and the gradient of a
+    k^2-weighted loss of a sub-sample of 1000 paths along two random directions (five-point differences of the
+    oracle) and FP32 against FP64 (max over atoms)."""
+    import oracle
+    from feffgpu_amd.compare import r_factor
+    from feffgpu_amd.paths import PathPlan
+    q = wl.q
+    path32 = plan32.forward(torch.tensor(wl.pos, device=dev))[1][torch.tensor(idx, device=dev)].double().cpu().numpy()
+    # per path: max|dchi_p| relative to the largest path of the sample (a path's own scale would weigh paths whose
+    # signal is a millionth of the largest, whose FP32 error is then relatively large and irrelevant)
+    per_path = float(np.abs(path32 - path_ref).max() / np.abs(path_ref).max())
+    pl = wl.pl
+    seg32 = np.zeros_like(seg_ref)
+    np.add.at(seg32, pl.segment[idx], pl.deg[idx][:, None] * path32)
+    live = np.abs(seg_ref).max(1) > 0
+    kmax = min(17.0, float(q[-1]) - 1.0)
+    R = max(r_factor(q, seg_ref[s], seg32[s], kmin=3.0, kmax=kmax, rmin=1.0, rmax=rmax) for s in np.nonzero(live)[0])
+    seg_rel = float(np.abs(seg32 - seg_ref).max() / np.abs(seg_ref).max())
+    # gradient: a sub-sample, its own plans (FP32 and FP64), the loss sum_s,q (q^2 chi_s(q) w_s(q))
+    rng = np.random.default_rng(6)
+    sub = np.sort(rng.choice(idx, min(1000, len(idx)), replace=False))
+    a = _oracle_args(wl, sub)
+    Wt = rng.normal(size=(wl.nseg, len(q))) * q ** 2
+    grads = {}
+    for prec in (64, 32):
+        plan = PathPlan([str(p) for p in wl.phases], q, a["atoms"], a["nleg"], a["absorber"], a["ipots"], a["deg"],
+                        segment=a["segment"], nseg=wl.nseg, edge=a["edge"], sigma2=a["sigma2"], cell=wl.cell,
+                        precision=prec, device=dev)
+        p = torch.tensor(wl.pos, device=dev, requires_grad=True)
+        seg, _ = plan(p)
+        (g,) = torch.autograd.grad((seg * torch.tensor(Wt, device=dev)).sum(), p)
+        grads[prec] = g.cpu().numpy()
+    phs = [oracle.OraclePhase(p) for p in wl.phases]
+
+    def L(x):
+        s, _, _ = oracle.chi_paths(phs, x, a["atoms"], a["nleg"], a["absorber"], a["ipots"], a["deg"], q,
+                                   segment=a["segment"], nseg=a["nseg"], edge=a["edge"], sigma2=a["sigma2"],
+                                   cell=a["cell"])
+        return float((s * Wt).sum())
+    h, fd_err = 1e-4, 0.0
+    for _ in range(2):
+        d = rng.normal(size=wl.pos.shape)
+        d /= np.linalg.norm(d)
+        fd = (-L(wl.pos + 2 * h * d) + 8 * L(wl.pos + h * d) - 8 * L(wl.pos - h * d) + L(wl.pos - 2 * h * d)) / (12 * h)
+        fd_err = max(fd_err, abs(float((grads[32] * d).sum()) - fd) / np.abs(grads[64]).sum())
+    g32 = float(np.abs(grads[32] - grads[64]).max() / np.abs(grads[64]).max())
+    return dict(sample_paths=int(len(idx)), per_path_max_abs_rel_to_largest=per_path, sample_segments_r_factor_max=float(R),
+                sample_segments_max_rel=seg_rel, grad_paths=int(len(sub)),
+                grad_fp32_vs_oracle_fd_directional=float(fd_err), grad_fp32_vs_fp64_max_rel=g32)
 
 
 def dominant_bucket(plan, pl):
     """The bucket with the most algorithmic work and the per-bucket flops (roofline.path_cmacs x 8)."""
     from feffgpu_amd.roofline import FLOPS_PER_CMAC, path_cmacs
-    cm = path_cmacs(plan.tables[0], plan.nleg_h, pl.ipots)
     work = []
     for b in range(len(plan.buckets.kind)):
         sel = plan.buckets.order[plan.buckets.offset[b]:plan.buckets.offset[b + 1]]
-        work.append(float(cm[sel].sum() * FLOPS_PER_CMAC))
+        T = plan.tables[int(plan.buckets.edge[b])]
+        work.append(float(path_cmacs(T, plan.nleg_h[sel], pl.ipots[sel]).sum() * FLOPS_PER_CMAC))
     return int(np.argmax(work)), work
 
 
@@ -181,12 +306,15 @@ def main():
         raise SystemExit("bench.py needs a ROCm GPU")
     dev = torch.device("cuda", 0)
     t_setup = time.perf_counter()
-    pos, pl, q, cell, nseg, cfg = build_workload(args.workload)
-    print(f"[bench] workload {args.workload}: {len(pl)} paths, {len(q)} k-points, setup {time.perf_counter() - t_setup:.1f} s",
-          file=sys.stderr, flush=True)
-    common = dict(segment=pl.segment, nseg=nseg, cell=cell, device=dev)
-    plan32 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
-    plan64 = PathPlan(str(PHASE_K), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, **common)
+    wl = build_workload(args.workload)
+    pos, pl, q, cell, nseg, cfg = wl.pos, wl.pl, wl.q, wl.cell, wl.nseg, wl.cfg
+    counts = {int(n): int((pl.nleg == n).sum()) for n in np.unique(pl.nleg)}
+    print(f"[bench] workload {args.workload}: {len(pl)} paths {counts}, {len(q)} k-points, setup "
+          f"{time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+    common = dict(segment=pl.segment, nseg=nseg, cell=cell, edge=wl.edge, sigma2=wl.sigma2, device=dev)
+    phases = [str(p) for p in wl.phases]
+    plan32 = PathPlan(phases, q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=32, **common)
+    plan64 = PathPlan(phases, q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, **common)
     pos_t = torch.tensor(pos, device=dev, dtype=torch.float64)
     sync = torch.cuda.synchronize
     units = len(pl) * len(q)
@@ -218,7 +346,10 @@ def main():
     s32 = plan32.forward(pos_t)[0].cpu().numpy()
     s64 = plan64.forward(pos_t)[0].cpu().numpy()
     from feffgpu_amd.compare import r_factor
-    Rs = [r_factor(q, s64[i], s32[i], kmin=3.0, kmax=17.0, rmin=1.0, rmax=5.0) for i in range(nseg)]
+    rmax = float(cfg["rmax_A"])
+    kmax_r = min(17.0, float(q[-1]) - 1.0)
+    Rs = [r_factor(q, s64[i], s32[i], kmin=3.0, kmax=kmax_r, rmin=1.0, rmax=rmax) for i in range(nseg)
+          if np.abs(s64[i]).max() > 0]
     maxrel = float(np.abs(s32 - s64).max() / np.abs(s64).max())
 
     ms_fwd32 = timed(lambda: plan32.forward(pos_t), args.steps, args.warmup, sync) * 1e3
@@ -247,11 +378,16 @@ def main():
     ms_fwd64 = timed(lambda: plan64.forward(pos_t), max(2, args.steps // 2), 1, sync) * 1e3
     print(f"[bench] fp64 fused forward {ms_fwd64:.3f} ms/step", file=sys.stderr, flush=True)
     # the comparator: the plain FP64 pipeline (the reference's op sequence, one kernel per stage, HBM intermediates)
-    plain_seg = plan64.forward_plain(pos_t)[0].cpu().numpy()
-    plain_rel = float(np.abs(plain_seg - s64).max() / np.abs(s64).max())
-    ms_plain64 = timed(lambda: plan64.forward_plain(pos_t), 2, 1, sync) * 1e3
-    print(f"[bench] fp64 plain forward {ms_plain64:.3f} ms/step (vs fused fp64 max rel {plain_rel:.1e})",
-          file=sys.stderr, flush=True)
+    # (two-leg paths of L and M edges have no plain kernel: such a workload reports no comparator)
+    try:
+        plain_seg = plan64.forward_plain(pos_t)[0].cpu().numpy()
+        plain_rel = float(np.abs(plain_seg - s64).max() / np.abs(s64).max())
+        ms_plain64 = timed(lambda: plan64.forward_plain(pos_t), 2, 1, sync) * 1e3
+        print(f"[bench] fp64 plain forward {ms_plain64:.3f} ms/step (vs fused fp64 max rel {plain_rel:.1e})",
+              file=sys.stderr, flush=True)
+    except NotImplementedError as e:
+        plain_rel = ms_plain64 = None
+        print(f"[bench] fp64 plain forward not available: {e}", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel of the step, the adjoint of the bucket with the most work (algorithmic flops
     # = 2 x the forward's, recomputation not counted), and of that bucket's forward; HIP events on the launch stream
@@ -269,7 +405,8 @@ def main():
         return ev[0].elapsed_time(ev[1]) / nrep
 
     kind_names = {0: "K multiple scattering", 1: "L/M multiple scattering", 2: "single scattering"}
-    what = f"{kind_names.get(int(plan32.buckets.kind[bdom]))}, {int(plan32.buckets.nleg[bdom])} legs"
+    what = f"{kind_names.get(int(plan32.buckets.kind[bdom]))}, {int(plan32.buckets.nleg[bdom])} legs, " \
+           f"{int(plan32.buckets.offset[bdom + 1] - plan32.buckets.offset[bdom])} paths"
     t_bwd = launch_ms(lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
     t_fwd = launch_ms(lambda: plan32.forward_bucket(pos_t, bdom))
     peak = MI355X_PEAK_TFLOPS["fp32"]
@@ -279,8 +416,10 @@ def main():
     ach_f = b_work[bdom] / (t_fwd * 1e-3) / 1e12
     nl = int(plan32.buckets.nleg[bdom])
     kset = int(plan32.buckets.kind[bdom]) if nl > 2 else 3 + plan32.tables[int(plan32.buckets.edge[bdom])].ilinit - 1
-    roof = dict(bound="mfma", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak,
-                traffic=measured_traffic("k_adjoint") if (kset, nl) == (0, 4) else None,
+    roof = dict(bound="valu", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak,
+                traffic=measured_traffic("k_adjoint") if (kset, nl) == (0, 4) and args.workload == "md" else None,
+                traffic_source="profiles/k_adjoint_md_batch.md (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-"
+                               "calibrated)" if (kset, nl) == (0, 4) and args.workload == "md" else None,
                 kernel=f"k_adjoint<float,{kset},{nl},*> + k_adjoint_tail<float,{nl}> ({what})", launch_ms=t_bwd,
                 flops_per_launch=2 * b_work[bdom], note=note)
     roof_fwd = dict(achieved=ach_f, peak=peak, unit="TFLOP/s", frac=ach_f / peak,
@@ -298,7 +437,7 @@ def main():
 
     # single scattering of whole frames from pair tables: every atom of every frame an absorber, pairs within 5 A
     pairs = None
-    if cell is not None:
+    if args.workload == "md":
         from feffgpu_amd.pairs import PairTables
         pt = PairTables(str(PHASE_K), q, box=cell, rmax=5.0, r_min=1.5, dr=0.002, ms=False, device=dev)
         Pf = pos_t.reshape(nseg, -1, 3).clone().requires_grad_(True)
@@ -321,16 +460,23 @@ def main():
                median_event_ms_per_step=med_fb32, min_event_ms_per_step=float(min(per)),
                fp32_fused_fwd=dict(value=units / (ms_fwd32 * 1e-3), ms_per_step=ms_fwd32),
                fp64_fused_fwd=dict(value=units / (ms_fwd64 * 1e-3), ms_per_step=ms_fwd64),
-               fp64_plain_fwd=dict(value=units / (ms_plain64 * 1e-3), ms_per_step=ms_plain64,
-                                   max_rel_vs_fused_fp64=plain_rel),
-               speedup_fp32_fused_fwd_bwd_vs_fp64_plain_fwd=ms_plain64 / ms_fb32,
+               fp64_plain_fwd=None if ms_plain64 is None else dict(value=units / (ms_plain64 * 1e-3),
+                                                                   ms_per_step=ms_plain64,
+                                                                   max_rel_vs_fused_fp64=plain_rel),
+               speedup_fp32_fused_fwd_bwd_vs_fp64_plain_fwd=None if ms_plain64 is None else ms_plain64 / ms_fb32,
                ratio_fp64_fused_over_fp32_fused_fwd=ms_fwd64 / ms_fwd32,
                parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
                                   backward_ms=policy["backward"]),
                roofline=roof, roofline_forward=roof_fwd, misfit=misfit, pair_tables=pairs)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pos, pl, q, cell, args.cpu_seconds)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        threads = max(1, min(threads, os.cpu_count() or threads))
+        base, idx, seg_ref, path_ref = cpu_baseline(wl, args.cpu_seconds, threads)
+        out["cpu_baseline"] = base
+        print(f"[bench] cpu baseline {base['value']:.3e} {base['unit']} ({base['sample']})", file=sys.stderr, flush=True)
+        out["parity_vs_oracle"] = oracle_parity(torch, wl, plan32, idx, seg_ref, path_ref, rmax, dev)
+        print(f"[bench] parity vs oracle {out['parity_vs_oracle']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

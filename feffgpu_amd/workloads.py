@@ -55,42 +55,36 @@ def displaced(pos, sigma, seed, keep_first=False):
     return out
 
 
-def random_packing(n: int, box: float, dmin: float, seed: int, max_tries: int = 200):
-    """Random sequential addition of n points in a cubic periodic box with minimum image separation dmin."""
+def random_packing(n: int, box: float, dmin: float, seed: int, max_rounds: int = 100000):
+    """Random sequential addition of n points in a cubic periodic box with minimum image separation dmin: uniform
+    candidates in batches, each kept if it is at least dmin from every point placed before it (a periodic k-d tree
+    for the points of earlier batches, a direct check within the batch, in the candidates' order)."""
+    from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
-    nc = max(1, int(box // dmin))
-    h = box / nc
-    cells = {}
-    pts = np.zeros((n, 3))
-    k = 0
-    for _ in range(n * max_tries):
-        if k == n:
-            break
-        p = rng.uniform(0, box, 3)
-        c = tuple((p // h).astype(int) % nc)
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for j in cells.get(((c[0] + dx) % nc, (c[1] + dy) % nc, (c[2] + dz) % nc), ()):
-                        d = pts[j] - p
-                        d -= box * np.round(d / box)
-                        if d @ d < dmin * dmin:
-                            ok = False
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if not ok:
+    pts = np.zeros((0, 3))
+    tree = None
+    for _ in range(max_rounds):
+        if len(pts) == n:
+            return pts
+        m = min(4 * (n - len(pts)) + 16, 20000)
+        cand = rng.uniform(0.0, box, (m, 3))
+        if tree is not None:
+            d, _ = tree.query(cand, k=1)
+            cand = cand[d >= dmin]
+        new = []
+        for c in cand:
+            if len(pts) + len(new) == n:
                 break
-        if ok:
-            pts[k] = p
-            cells.setdefault(c, []).append(k)
-            k += 1
-    if k < n:
-        raise RuntimeError(f"random packing placed only {k} of {n} atoms")
-    return pts
+            if new:
+                v = np.asarray(new) - c
+                v -= box * np.round(v / box)
+                if (np.einsum("ij,ij->i", v, v) < dmin * dmin).any():
+                    continue
+            new.append(c)
+        if new:
+            pts = np.concatenate([pts, np.asarray(new)])
+            tree = cKDTree(np.mod(pts, box), boxsize=box)
+    raise RuntimeError(f"random packing placed only {len(pts)} of {n} atoms")
 
 
 def _wrap(v, box):

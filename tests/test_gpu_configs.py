@@ -115,3 +115,87 @@ def test_multi_edge_two_to_six_legs():
     target = oracle.chi_paths(orc, hidden, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, Q, segment=pl.segment,
                               nseg=2, edge=edge, cell=box)[0]
     _check(ph, pos, pl, 2, box, edge, target, "multi-edge")
+
+
+def _cu_cluster():
+    """fcc Cu cluster of radius 6 A (a = 3.61 A, 79 atoms), every atom displaced by sigma = 0.03 A (seeded), re-centred
+    on the absorber; nleg <= 4, R_max 5 A; the W K-edge phase file stands in for Cu K (no Cu phase file ships)."""
+    cl = W.cluster("fcc", 3.61, 6.0)
+    assert len(cl) == 79
+    pos = W.displaced(cl, 0.03, 101)
+    pos = pos - pos[0]
+    pl = W.enumerate_paths(pos, 0, np.ones(len(pos), int), 5.0, 4, nmax=4)
+    return GOLDEN / "W" / "phase.pad", pos, pl
+
+
+@pytest.mark.parametrize("prec,tol", [(64, 1e-10), (32, 1e-5)], ids=["fp64", "fp32"])
+def test_config_fcc_cu_cluster_full_size(prec, tol):
+    """The CPU-plain configuration at its full size (k 0-16 step 0.05, 321 points, sigma^2 = 0.003 A^2 on every
+    path): the fused kernels and the plain FP64 pipeline against the oracle, R-factor and max|dchi|/max|chi|."""
+    from feffgpu_amd.compare import r_factor
+    from feffgpu_amd.paths import PathPlan
+    phase, pos, pl = _cu_cluster()
+    q = np.arange(0.0, 16.0001, 0.05)
+    assert len(q) == 321
+    ref, _, _ = oracle.chi_paths(oracle.OraclePhase(phase), pos, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, q,
+                                 sigma2=0.003)
+    plan = PathPlan(str(phase), q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, sigma2=0.003, precision=prec,
+                    device="cuda")
+    seg = plan.forward(torch.tensor(pos, device="cuda"))[0].cpu().numpy()[0]
+    err = float(np.abs(seg - ref[0]).max() / np.abs(ref[0]).max())
+    R = r_factor(q, ref[0], seg, kmin=3.0, kmax=15.0, rmin=1.0, rmax=5.0)
+    print(f"[config cu] FP{prec}: {len(pl)} paths x 321 k: max rel {err:.2e}, R {R:.2e}", flush=True)
+    assert err < tol and R < (1e-12 if prec == 64 else 2e-5)
+    if prec == 64:
+        plain = plan.forward_plain(torch.tensor(pos, device="cuda"))[0].cpu().numpy()[0]
+        assert np.abs(plain - ref[0]).max() < 1e-10 * np.abs(ref[0]).max()
+
+
+def test_config_bcc_fe_cluster_plain_full_size():
+    """The plain-FP64 configuration at its full size: bcc cluster of radius 9 A (a = 2.87 A, 259 atoms), sigma
+    0.05 A, nleg <= 4, R_max 6 A, k 0-18 step 0.05 (1577 paths: every path within R_max, no importance filter); the
+    plain FP64 pipeline and the fused FP64 kernel against the
+    oracle on every path (A018's V K edge stands in for Fe K)."""
+    from feffgpu_amd.compare import r_factor
+    from feffgpu_amd.paths import PathPlan
+    phase = GOLDEN / "validation" / "A018" / "phase.pad"
+    cl = W.cluster("bcc", 2.87, 9.0)
+    assert len(cl) == 259
+    pos = W.displaced(cl, 0.05, 7, keep_first=True)
+    pl = W.enumerate_paths(pos, 0, np.full(len(pos), 2), 6.0, 4, nmax=4)
+    assert len(pl) == 1577                    # 64 two-leg, 408 three-leg, 1105 four-leg paths (no importance filter)
+    ref, ref_path, _ = oracle.chi_paths(oracle.OraclePhase(phase), pos, pl.atoms, pl.nleg, pl.absorber, pl.ipots,
+                                        pl.deg, Q, want_paths=True)
+    plan = PathPlan(str(phase), Q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, precision=64, device="cuda")
+    x = torch.tensor(pos, device="cuda")
+    plain, plain_path, _ = (t.cpu().numpy() for t in plan.forward_plain(x))
+    fused = plan.forward(x)[0].cpu().numpy()
+    for name, s in (("plain", plain), ("fused", fused)):
+        err = float(np.abs(s[0] - ref[0]).max() / np.abs(ref[0]).max())
+        R = r_factor(Q, ref[0], s[0], kmin=3.0, kmax=17.0, rmin=1.0, rmax=6.0)
+        print(f"[config fe] {name} FP64: {len(pl)} paths: max rel {err:.2e}, R {R:.2e}", flush=True)
+        assert err < 1e-10 and R < 1e-15
+    rel = np.abs(plain_path - ref_path).max(1) / np.abs(ref_path).max(1)
+    assert rel.max() < 1e-9
+
+
+def test_config_nio_refinement_full_box():
+    """The refinement configuration on its full box: rocksalt NiO of 10 x 10 x 5 cells (4000 atoms, a = 4.18 A), a
+    seeded perturbation sigma = 0.1 A, the target chi from a hidden structure (another seed), k^2-weighted L2 loss,
+    nleg <= 4, R_max 5 A; four Ni absorbers (a stated subset of the 2000) so that the oracle's finite differences
+    stay in seconds. A018 (V K edge in Ge-V rock salt: potential 2 on the metal, 1 on the anion) stands in for Ni K
+    in NiO."""
+    phase = GOLDEN / "validation" / "A018" / "phase.pad"
+    pos0, species, box = W.rocksalt(4.18, (10, 10, 5))
+    assert len(pos0) == 4000
+    ipot = np.where(species == 0, 2, 1)
+    pos = W.displaced(pos0, 0.1, 41)
+    hidden = W.displaced(pos0, 0.1, 42)
+    absorbers = np.nonzero(species == 0)[0][[3, 500, 1234, 1999]]
+    pl = W.concat([W.enumerate_paths(pos, int(a), ipot, 5.0, 4, box=box, segment=i, nmax=4)
+                   for i, a in enumerate(absorbers)], 4)
+    orc = oracle.OraclePhase(phase)
+    cell = np.diag(box)
+    target = oracle.chi_paths(orc, hidden, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, Q, segment=pl.segment,
+                              nseg=len(absorbers), cell=cell)[0]
+    _check([phase], pos, pl, len(absorbers), cell, None, target, "config nio")
