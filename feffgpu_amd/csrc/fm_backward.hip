@@ -165,7 +165,7 @@ __device__ __forceinline__ void fblock_bwd(const cx<T> (&K)[FM_NREP], cx<T> ck, 
                                 Wdl + (size_t)r * FM_NREP, gX, gY, gB);
         t = tn;
     }
-#pragma unroll 1
+#pragma unroll 2
     for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa, xd);
         Y.step(r, ya, yd);
@@ -620,7 +620,7 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
         t = tn;
         tn = tnn;
     }
-#pragma unroll 1
+#pragma unroll 2
     for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa, xd);
         Y.step(r, ya, yd);
@@ -696,11 +696,12 @@ __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, 
     row_phases<T, SET>(ph, g, S + 1);
     cx<T> K[FM_NREP];
     {
-        cx<T> Kp[FM_NREP];
         cx<T> P[L][L0];
         if constexpr (S >= 2) prefix<T, SET, NL, S>(g, F, P);
 #pragma unroll
-        for (int k = 0; k < FM_NREP; ++k) K[k] = Kp[k] = mk<T>(T(0), T(0));
+        for (int k = 0; k < FM_NREP; ++k) K[k] = mk<T>(T(0), T(0));
+        // K from Pre_S and U; the eta_{S+1} derivative -i sum_{l1,l2} M[l1] h(l1, l2) F_S(l1, l2) on the way
+        cx<T> de = mk<T>(T(0), T(0));
 #pragma unroll
         for (int l2 = 0; l2 < NC; ++l2) {
 #pragma unroll
@@ -715,14 +716,9 @@ __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, 
                 const cx<T> h = m * ph[l1];
                 if (PD.sg[l1][l2] > 0) K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] + h;
                 else K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] - h;
-                if (D.M[l1] != 0) rfma(Kp[PD.idx[l1][l2]], h, T(PD.sg[l1][l2] * D.M[l1]));
+                if (D.M[l1] != 0) cfma(de, h * T(PD.sg[l1][l2] * D.M[l1]), F[S - 1][PD.idx[l1][l2]]);
             }
         }
-        // eta_{S+1}: -i sum_k Kp[k] F_S[k]
-        cx<T> de = mk<T>(T(0), T(0));
-#pragma unroll
-        for (int k = 0; k < FM_NREP; ++k)
-            if (PD.ri[k] < NR && PD.rj[k] < NC) cfma(de, Kp[k], F[S - 1][k]);
         ds.put(GK_H, S + 1, mk<T>(de.i, -de.r));
         if constexpr (S > 1) {
 #pragma unroll

@@ -345,7 +345,8 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
         t = tn;
         tn = tnn;
     }
-#pragma unroll 1
+    // two rows per iteration: the recursions' two-row state is renamed instead of copied every row
+#pragma unroll 2
     for (int r = 5; r <= ilx; ++r) {
         X.step(r, xa);
         Y.step(r, yb);
