@@ -101,7 +101,7 @@ template <class T, int SET> __host__ void layout(BArgs &a, const fm_edge &ed, bo
     const size_t gsb = sizeof(double) * 3 * (size_t)a.nlegmax * a.nlegmax;
     if (2 * tab >= gsb) a.off_gs = a.off_W;
     else { a.off_gs = o; o += align16(gsb); }
-    a.path_bytes = o;
+    a.path_bytes = path_stride(o);
     const int nslots = a.nlegmax > 2 ? a.nlegmax - 1 : 1;
     const bool store = SET < FM_SET_SS;   // single scattering keeps nothing but the gradient slots
     a.work_off_F = store ? sizeof(cx<T>) * (size_t)nslots * pre_elems(SET) * FM_WAVE : 0;
@@ -565,10 +565,10 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
 // from the kept F) and U_s = T F_{n-1} ... F_{s+1}, the eta derivative, U_{s-1}, and the tangent stream of F_s.
 
 // One partial-wave row of a block's tangents contracted with K, accumulated as complex derivatives.
-template <class T, int SET, int NR, int NC>
+template <class T, int SET, int NR, int NC, class WRow>
 __device__ __forceinline__ void bwd_row_c(const cx<T> (&K)[FM_NREP], const cx<T> (&xa)[3], const cx<T> (&xd)[3],
-                                          cx<T> y0, cx<T> y1, cx<T> y0d, cx<T> y1d, const T *__restrict__ w,
-                                          const T *__restrict__ wd, cx<T> &dX, cx<T> &dY, cx<T> &dB) {
+                                          cx<T> y0, cx<T> y1, cx<T> y0d, cx<T> y1d, const WRow &w,
+                                          const WRow &wd, cx<T> &dX, cx<T> &dY, cx<T> &dB) {
     constexpr PairDef PD = pairdef(SET);
     cx<T> Z[3][2], Q[3][2], Qd[3][2];
 #pragma unroll
@@ -620,17 +620,27 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
         t = tn;
         tn = tnn;
     }
-#pragma unroll 2
-    for (int r = 5; r <= ilx; ++r) {
+    // as in fblock: the row's table values read before the recursion steps that hide them, two rows per iteration
+    auto row = [&](int r) {
+        T wv[FM_NREP], wdv[FM_NREP];
+        load_wrow<T, SET, NR, NC>(wv, Wl + (size_t)r * FM_NREP);
+        load_wrow<T, SET, NR, NC>(wdv, Wdl + (size_t)r * FM_NREP);
+        __builtin_amdgcn_sched_barrier(0);
         X.step(r, xa, xd);
         Y.step(r, ya, yd);
         const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
         const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
-        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
-                                  Wdl + (size_t)r * FM_NREP, dX, dY, dB);
+        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, wv, wdv, dX, dY, dB);
         t = tn;
         tn = tnn;
+    };
+    int r = 5;
+#pragma unroll 1
+    for (; r < ilx; r += 2) {
+        row(r);
+        row(r + 1);
     }
+    if (r == ilx) row(r);
 }
 
 // Complex derivative slots of a lane, (kind, leg j) -> slot kind nleg + j - 1, in the workspace region of the lane's
@@ -770,9 +780,9 @@ __device__ __forceinline__ void fused_fblocks(const fm_edge &ed, const PathLDS &
     const int ilx = ed.lmax[(size_t)ip * E + ee] + 1;
     const T *tl = TL<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
     const T *Wl = W + (size_t)(S - 1) * ltab * FM_NREP;
-    if constexpr (S == 1) fblock<T, SET, L, L0>(F[0], ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, 2 * E);
-    else if constexpr (S == NL - 1) fblock<T, SET, L0, L>(F[S - 1], ck, (T)g.r[S + 1], (T)g.r[S], ilx, Wl, tl, 2 * E);
-    else fblock<T, SET, L, L>(F[S - 1], ck, (T)g.r[S + 1], (T)g.r[S], ilx, Wl, tl, 2 * E);
+    if constexpr (S == 1) fblock<T, SET, L, L0, true>(F[0], ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, 2 * E);
+    else if constexpr (S == NL - 1) fblock<T, SET, L0, L, true>(F[S - 1], ck, (T)g.r[S + 1], (T)g.r[S], ilx, Wl, tl, 2 * E);
+    else fblock<T, SET, L, L, true>(F[S - 1], ck, (T)g.r[S + 1], (T)g.r[S], ilx, Wl, tl, 2 * E);
     if constexpr (S < NL - 1) fused_fblocks<T, SET, NL, S + 1>(ed, g, W, ltab, ee, F);
 }
 
@@ -1520,7 +1530,7 @@ template <class T, int SET> __host__ void layout_sweep(BArgs &a) {
     const size_t tab = SET >= FM_SET_SS ? 0 : align16(sizeof(T) * rot_elems(a.nlegmax, a.ltab));
     a.off_W = o; o += tab;
     a.off_Wd = o; o += tab;
-    a.path_bytes = o;
+    a.path_bytes = path_stride(o);
 }
 
 // LDS of one wavefront (one path) of k_adjoint_tail.

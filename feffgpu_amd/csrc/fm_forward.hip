@@ -191,8 +191,9 @@ int set_for(const fm_edge &ed, int nleg) {
 // largest that fits one workgroup.
 template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st) {
     const fm_edge &ed = a.ed;
-    a.path_bytes = align16(sizeof(PathLDS)) + (SET < FM_SET_SS ? align16(sizeof(T) * rot_elems(a.nleg, ed.L1 + 1)) : 0) +
-                   align16(sizeof(T) * ed.E) + align16(sizeof(double) * ed.E);
+    a.path_bytes = path_stride(align16(sizeof(PathLDS)) +
+                               (SET < FM_SET_SS ? align16(sizeof(T) * rot_elems(a.nleg, ed.L1 + 1)) : 0) +
+                               align16(sizeof(T) * ed.E) + align16(sizeof(double) * ed.E));
     int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 0 : 2) - 256);
     if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
     if (g < 0) return hipErrorInvalidValue;

@@ -41,6 +41,13 @@ struct PathLDS {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+// The LDS stride between the paths of a workgroup: an odd multiple of 16 bytes, so that the up to 8 paths whose
+// lanes share a wavefront read the same table offset from 8 different groups of banks.
+__host__ __device__ inline size_t path_stride(size_t bytes) {
+    const size_t s = align16(bytes);
+    return (s / 16) % 2 ? s : s + 16;
+}
+
 // Elements of the rotation tables of a path of at most nl legs: legs 1..nl-1 hold ltab rows each; the last leg
 // (the termination's) is read at rows 1..4 only, and is stored with 5 rows at leg offset n - 1 <= nl - 1.
 __host__ __device__ inline size_t rot_elems(int nl, int ltab) { return ((size_t)(nl - 1) * ltab + 5) * FM_NREP; }
@@ -302,9 +309,17 @@ __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path
 
 // One partial-wave row of a scattering matrix: Ft[k] += X_a(k) Y_b(k) w[k] for the representatives k of the
 // NR x NC block (Y already carries the t-matrix).
+// The rotation-table values of one row for the representatives of an NR x NC block, LDS -> registers.
 template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void load_wrow(T (&v)[FM_NREP], const T *__restrict__ w) {
+    constexpr PairDef PD = pairdef(SET);
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k) v[k] = (PD.ri[k] < NR && PD.rj[k] < NC) ? w[k] : T(0);
+}
+
+template <class T, int SET, int NR, int NC, class WRow>
 __device__ __forceinline__ void accum_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&xa)[3], cx<T> y0, cx<T> y1,
-                                          const T *__restrict__ w) {
+                                          const WRow &w) {
     constexpr PairDef PD = pairdef(SET);
     cx<T> Z[3][2];
 #pragma unroll
@@ -321,7 +336,9 @@ __device__ __forceinline__ void accum_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&xa
 // scatterer's t-matrix, summed over il <= ilx with the rotation table of leg s (src/feffgpu/stage.py:287-305).
 // Accumulates the representatives k with ri < NR, rj < NC into Ft (zeroed here). Each lane runs its own rows
 // il = 1..ilx (the wave runs to the largest ilx of its lanes); the t-matrix is fetched two rows ahead.
-template <class T, int SET, int NR, int NC>
+// PREFETCH: each row's table values are read into registers before its recursion steps (the adjoint's sweep, at one
+// wavefront per SIMD, has no second wavefront to hide the LDS latency); otherwise where the compiler puts them.
+template <class T, int SET, int NR, int NC, bool PREFETCH = false>
 __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
                                        const T *__restrict__ Wl, const T *__restrict__ tl, int tlstride) {
 #pragma unroll
@@ -345,16 +362,38 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
         t = tn;
         tn = tnn;
     }
-    // two rows per iteration: the recursions' two-row state is renamed instead of copied every row
-#pragma unroll 2
-    for (int r = 5; r <= ilx; ++r) {
-        X.step(r, xa);
-        Y.step(r, yb);
-        const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
-        const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
-        accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
-        t = tn;
-        tn = tnn;
+    if constexpr (PREFETCH) {
+        // two rows per iteration, so that the recursions' two-row state is renamed instead of copied every row
+        auto row = [&](int r) {
+            const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
+            T wv[FM_NREP];
+            load_wrow<T, SET, NR, NC>(wv, Wl + (size_t)r * FM_NREP);
+            __builtin_amdgcn_sched_barrier(0);
+            X.step(r, xa);
+            Y.step(r, yb);
+            const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
+            accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, wv);
+            t = tn;
+            tn = tnn;
+        };
+        int r = 5;
+#pragma unroll 1
+        for (; r < ilx; r += 2) {
+            row(r);
+            row(r + 1);
+        }
+        if (r == ilx) row(r);
+    } else {
+#pragma unroll 1
+        for (int r = 5; r <= ilx; ++r) {
+            X.step(r, xa);
+            Y.step(r, yb);
+            const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
+            const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
+            accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
+            t = tn;
+            tn = tnn;
+        }
     }
 }
 
