@@ -56,6 +56,23 @@ class FmPairGrid(ctypes.Structure):
                 ("cell_kind", _i32), ("cell", ctypes.c_double * 9), ("cellinv", ctypes.c_double * 9)]
 
 
+class FmFindSys(ctypes.Structure):
+    _fields_ = [("natom", _i32), ("pos", _vp), ("cell_kind", _i32), ("cell", ctypes.c_float * 9),
+                ("cellinv", ctypes.c_float * 9), ("rmax", ctypes.c_float)]
+
+
+class FmFindTables(ctypes.Structure):
+    _fields_ = [("E", _i32), ("nph1", _i32), ("ik0", _i32), ("fbeta", _vp), ("cksp", _vp), ("xlam", _vp), ("trap", _vp)]
+
+
+class FmFindOut(ctypes.Structure):
+    _fields_ = [("atoms", _vp), ("coords", _vp), ("ipots", _vp), ("deg", _vp), ("slot", _vp)]
+
+
+class FmFindPartial(ctypes.Structure):
+    _fields_ = [("seq", _vp), ("len", _vp), ("slot", _vp)]
+
+
 def grid_runs(q, jq_rows, E_list):
     """qorder (q indices in ascending q) and qstart [nedge][stride]: for every edge, the first position in qorder
     whose lower interpolation index is >= e, for e = 0..E (the runs the adjoint of the fine grid gathers)."""
@@ -95,6 +112,25 @@ def lib():
                                        ctypes.c_size_t, _vp]
         L.fm_cchi_backward_noalloc.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
                                                _vp, _vp]
+        _i64, _f32, _sz = ctypes.c_int64, ctypes.c_float, ctypes.c_size_t
+        _P = ctypes.POINTER
+        L.fm_find_candidates.argtypes = [_P(FmFindSys), _i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]
+        L.fm_find_count.argtypes = [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _f32, _i32, _vp, _vp, _vp]
+        L.fm_scan_workspace_bytes.restype = _sz
+        L.fm_scan_workspace_bytes.argtypes = [_i64]
+        L.fm_exclusive_scan.argtypes = [_vp, _i64, _vp, _vp, _vp, _sz, _vp]
+        L.fm_find_write.argtypes = [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _f32, _i32, _vp,
+                                    _P(FmFindOut), _vp, _P(FmFindOut), _vp, _P(FmFindOut), _vp]
+        L.fm_find_seed.argtypes = [_i32, _vp, _i32, _vp, _vp, _i32, _P(FmFindPartial), _vp]
+        L.fm_find_extend.argtypes = [_i64, _i32, _P(FmFindPartial), _i32, _vp, _vp, _vp, _f32, _vp, _vp,
+                                     _P(FmFindPartial), _vp]
+        L.fm_find_emit.argtypes = [_i64, _i32, _P(FmFindPartial), _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _P(FmFindOut), _vp]
+        L.fm_find_importance.argtypes = [_P(FmFindTables), _i64, _i32, _vp, _vp, _vp, _vp, _vp]
+        L.fm_find_group_workspace_bytes.restype = _sz
+        L.fm_find_group_workspace_bytes.argtypes = [_i64]
+        L.fm_find_degenerate_total.argtypes = [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _sz, _vp]
+        L.fm_find_filter.argtypes = [_i64, _vp, _vp, _vp, _vp, _f32, _vp, _vp]
         L.fm_backward_workspace_bytes.restype = ctypes.c_size_t
         L.fm_backward_workspace_bytes.argtypes = [_i32, _i32, _i32]
         L.fm_chi_fine.argtypes = [ctypes.POINTER(FmEdge), _i32, _i32, _vp, _vp, ctypes.POINTER(FmGrid), _vp, _vp]

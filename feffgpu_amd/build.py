@@ -11,7 +11,7 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SOURCES = ["csrc/fm_forward.hip", "csrc/fm_backward.hip", "csrc/fm_plain.hip", "csrc/fm_loss.hip", "csrc/fm_pairs.hip"]
+SOURCES = ["csrc/fm_forward.hip", "csrc/fm_backward.hip", "csrc/fm_plain.hip", "csrc/fm_loss.hip", "csrc/fm_pairs.hip", "csrc/fm_finder.hip"]
 OUT = HERE / "libfeffgpu_amd.so"
 ARCH = os.environ.get("FEFFGPU_AMD_ARCH", "gfx950")
 

@@ -231,6 +231,95 @@ int fm_pair_histogram(int32_t nconf, int32_t natom, const double *pos, int32_t n
 int fm_pair_histogram_grad(int32_t nconf, int32_t natom, const double *pos, const int32_t *ipot, int32_t npot,
                            const fm_pair_grid *grid, const double *gbin, double *grad_pos, void *stream);
 
+/* ---------------------------------------------------------------- path finder (src/feffgpu/pathfinder.py) --------
+ * FEFF's path finder for absorbers of one configuration, float32 as the reference: GPUPathFinder.enumerate
+ * (pathfinder.py:211-279), importance (:72-94), degenerate_total (:135-158) and the importance filter (:266-279).
+ * The enumeration runs in two passes: counts per work item, an exclusive scan (fm_exclusive_scan), then the paths
+ * written at the scanned offsets; feffgpu_amd/pathfinder.py drives it. */
+typedef struct fm_find_sys {
+    int32_t natom;
+    const float *pos;            /* [natom][3] A */
+    int32_t cell_kind;           /* 0 cluster, 1 cubic (cell[0] the length), 2 general (rows the lattice vectors) */
+    float cell[9];
+    float cellinv[9];
+    float rmax;                  /* A: half the largest total path length */
+} fm_find_sys;
+
+typedef struct fm_find_tables {
+    int32_t E, nph1, ik0;        /* energies, potentials, first energy of the importance integral (0-based) */
+    const float *fbeta;          /* [E][nph1][81] |f(beta)| on cos(beta) = -1, -0.975 .. 0.975, 1 */
+    const float *cksp;           /* [E] Re ck (1/A) */
+    const float *xlam;           /* [E] mean free path 1 / Im ck (A; 1e10 where Im ck vanishes) */
+    const float *trap;           /* [E - ik0] trapezoid weights over cksp[ik0:] */
+} fm_find_tables;
+
+typedef struct fm_find_out {     /* paths of one number of legs n */
+    int64_t *atoms;              /* [P][n - 1] global atom indices after the absorber */
+    float *coords;               /* [P][n][3] relative to the absorber (minimum image), absorber first at 0 */
+    int64_t *ipots;              /* [P][n] potential per atom, 0 for the absorber */
+    float *deg;                  /* [P] 2 for a path and its reverse, 1 for a palindrome */
+    int32_t *slot;               /* [P] which absorber of the call */
+} fm_find_out;
+
+typedef struct fm_find_partial { /* partial paths of the 5+-leg extension */
+    int16_t *seq;                /* [P][FM_MAX_LEGS - 1] positions in the absorber's candidate list (nc = absorber) */
+    float *len;                  /* [P] A, from the absorber */
+    int32_t *slot;               /* [P] */
+} fm_find_partial;
+
+/* Candidates of each absorber (atoms other than it within rmax, minimum image, in index order): cidx [nabs][maxc],
+ * crel [nabs][maxc][3], cd0 [nabs][maxc], ncand [nabs] (a count above maxc means the list was cut: call again with a
+ * larger maxc). The candidate selection of GPUPathFinder.enumerate (pathfinder.py:229-236). */
+int fm_find_candidates(const fm_find_sys *sys, int32_t nabs, const int32_t *absorbers, int32_t maxc, int32_t *cidx,
+                       float *crel, float *cd0, int32_t *ncand, void *stream);
+
+/* Counts of 3-leg paths per (absorber b, first atom i) at cnt3[off3[b] + i], and of 4-leg paths per (b, i,
+ * intermediate k) at cnt4[off4[b] + i (nc_b + 1) + k] (pathfinder.py:240-262); ncmax = the largest nc. */
+int fm_find_count(int32_t nabs, const int64_t *off3, const int64_t *off4, int32_t maxc, const float *crel,
+                  const float *cd0, const int32_t *ncand, int32_t ncmax, float rmax, int32_t nlegmax, int32_t *cnt3,
+                  int32_t *cnt4, void *stream);
+
+/* out[i] = sum of in[0..i) (int64) and *total = the sum of all n, for n <= 2^24; workspace of
+ * fm_scan_workspace_bytes(n). */
+size_t fm_scan_workspace_bytes(int64_t n);
+int fm_exclusive_scan(const int32_t *in, int64_t n, int64_t *out, int64_t *total, void *workspace, size_t bytes,
+                      void *stream);
+
+/* The paths of 2, 3 and 4 legs: two-leg paths of absorber b from off2[b]; 3- and 4-leg paths at the scanned item
+ * offsets pos3 / pos4, in the reference's order. */
+int fm_find_write(int32_t nabs, const int32_t *absorbers, const int64_t *off3, const int64_t *off4, int32_t maxc,
+                  const int32_t *cidx, const float *crel, const float *cd0, const int32_t *ncand, int32_t ncmax,
+                  const int32_t *ipot, float rmax, int32_t nlegmax, const int64_t *off2, const fm_find_out *o2,
+                  const int64_t *pos3, const fm_find_out *o3, const int64_t *pos4, const fm_find_out *o4, void *stream);
+
+/* Paths of 5+ legs (pathfinder.py:160-209): the seed partials (one atom: every candidate, from off2[b]); one more
+ * atom per partial (counts, or with pos the new partials); the completed paths of m + 1 legs among partials of m >= 4
+ * atoms (counts, or with pos the paths). */
+int fm_find_seed(int32_t nabs, const int64_t *off2, int32_t maxc, const float *cd0, const int32_t *ncand,
+                 int32_t ncmax, const fm_find_partial *out, void *stream);
+int fm_find_extend(int64_t npart, int32_t m, const fm_find_partial *in, int32_t maxc, const float *crel,
+                   const float *cd0, const int32_t *ncand, float rmax, int32_t *cnt, const int64_t *pos,
+                   const fm_find_partial *out, void *stream);
+int fm_find_emit(int64_t npart, int32_t m, const fm_find_partial *in, const int32_t *absorbers, int32_t maxc,
+                 const int32_t *cidx, const float *crel, const float *cd0, const int32_t *ncand, const int32_t *ipot,
+                 int32_t *cnt, const int64_t *pos, const fm_find_out *o, void *stream);
+
+/* GPUPathFinder.importance (pathfinder.py:72-94): xport [npath] of paths of nleg legs, coords [npath][nleg][3]
+ * (absorber first), ipots [npath][nleg]; rtot [npath] (or NULL) the total length. */
+int fm_find_importance(const fm_find_tables *t, int64_t npath, int32_t nleg, const float *coords, const int64_t *ipots,
+                       float *xport, float *rtot, void *stream);
+
+/* GPUPathFinder.degenerate_total (pathfinder.py:135-158) per absorber slot: total [npath] = the sum of deg over the
+ * paths of the same slot whose leg lengths and angles round to the same 1e-3 in either direction (a 64-bit hash of
+ * that key identifies the group). Workspace of fm_find_group_workspace_bytes(npath). */
+size_t fm_find_group_workspace_bytes(int64_t npath);
+int fm_find_degenerate_total(int64_t npath, int32_t nleg, const float *coords, const float *deg, const int32_t *slot,
+                             float *total, void *workspace, size_t bytes, void *stream);
+
+/* The importance filter (pathfinder.py:272-278): keep[p] = 100 dtot xp / ref[slot] >= critpw, in float32. */
+int fm_find_filter(int64_t npath, const float *dtot, const float *xport, const int32_t *slot, const float *ref,
+                   float critpw, uint8_t *keep, void *stream);
+
 /* Launch policy (process-wide): the LDS in KiB (16..160) that a workgroup may take when the number of paths per
  * workgroup is chosen, for kernel 0 (FP32 forward calls), 1 (FP32 adjoints), 2 (FP64 forward), 3 (FP64 adjoints);
  * defaults 80 (two FP32 forward workgroups per CU), then 160. Set by feffgpu_amd.tune from timings. Returns the

@@ -48,6 +48,7 @@ def lib():
         L.orc_tables.argtypes = [ctypes.c_void_p, _ip, _dp, _dp]
         L.orc_cchi.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _lp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.orc_wigner.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_phase_shifts.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
         L.orc_chi_fine.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_chi_paths.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip,
                                     _ip, _ip, _lp, _dp, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
@@ -92,6 +93,14 @@ class OraclePhase:
         ck = np.zeros(2 * self.ne1)
         lib().orc_tables(self.h, _p(lm, _ip), _p(xk, _dp), _p(ck, _dp))
         return lm, xk, ck[0::2] + 1j * ck[1::2]
+
+    def phase_shifts(self, sign=-1):
+        """Phase shifts of spin 0, (ne1, ltot + 1, nph + 1) complex: ph(ie, -l, iph) for sign < 0 (as FEFF's path
+        finder tabulates them), ph(ie, l, iph) otherwise; and ik0 (1-based, as in phase.pad)."""
+        out = np.zeros((self.ne1, self.ltot + 1, self.nph + 1, 2))
+        ik0 = np.zeros(1, np.int32)
+        lib().orc_phase_shifts(self.h, int(sign), _p(out, _dp), _p(ik0, _ip))
+        return out[..., 0] + 1j * out[..., 1], int(ik0[0])
 
     def spline_matrix(self, q):
         """Cubic-spline interpolation matrix (len(q), ne1) from the phase grid (1/A) to q, q clamped to the grid:

@@ -290,6 +290,23 @@ int orc_tables(const orc_phase *P, int *lmax /* ne1*(nph+1) */, double *xk /* ne
     return 0;
 }
 
+/* The phase shifts of spin 0, ph_out[((ie * (ltot + 1) + l) * (nph + 1) + iph) * 2 + re/im] for l = -ltot side
+ * only when sign < 0 (the j = l - 1/2 shifts FEFF's path finder tabulates, ph(ie, -l, iph)), else l >= 0; and ik0. */
+int orc_phase_shifts(const orc_phase *P, int sign, double *ph_out, int *ik0) {
+    const int nlt = 2 * P->ltot + 1;
+    for (int ie = 0; ie < P->ne1; ++ie)
+        for (int l = 0; l <= P->ltot; ++l)
+            for (int iph = 0; iph <= P->nph; ++iph) {
+                const int lo = sign < 0 ? -l : l;
+                cplx v = P->ph[(((size_t)ie * nlt + lo + P->ltot) * P->nsp + 0) * (P->nph + 1) + iph];
+                size_t o = (((size_t)ie * (P->ltot + 1) + l) * (P->nph + 1) + iph) * 2;
+                ph_out[o] = creal(v);
+                ph_out[o + 1] = cimag(v);
+            }
+    *ik0 = P->ik0;
+    return 0;
+}
+
 /* ---------------------------------------------------------------- path geometry ------------------------------ */
 static void trig(const double v[3], double *ct, double *st, double *cp, double *sp) {
     double x = v[0], y = v[1], z = v[2];
