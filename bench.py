@@ -52,22 +52,50 @@ def build_workload(name: str) -> Workload:
     from feffgpu_amd import workloads as W
     q = np.round(np.arange(0, 18.0 + 1e-9, 0.05), 10)
     if name == "md":
+        # the path lists of every frame from the GPU path finder with FEFF's plane-wave filter (critpw 2 %, the
+        # reference's MultipleScattering default); the finder's time is reported beside the stage's
+        import torch
+        from feffgpu_amd.pathfinder import GPUPathFinder
         pos0, box = W.lattice("bcc", 2.87, 10)
         n = len(pos0)
-        frames, nabs = 64, 16
+        frames, nabs, critpw = 64, 16, 2.0
         ipot = np.full(n, 2)
         rng = np.random.default_rng(2024)
-        pos_all, lists = [], []
-        for f in range(frames):
-            pos = W.displaced(pos0, 0.07, (2024, f))
-            pos_all.append(pos)
-            for a in rng.choice(n, nabs, replace=False):
-                lists.append(W.enumerate_paths(pos, int(a), ipot, 5.0, 4, box=box, offset=f * n, segment=f, nmax=4))
+        frames_pos = [W.displaced(pos0, 0.07, (2024, f)) for f in range(frames)]
+        absorbers = [rng.choice(n, nabs, replace=False) for _ in range(frames)]
+        finder = GPUPathFinder(str(PHASE_K), device="cuda", rmax=5.0, nlegmax=4, critpw=critpw,
+                               merge_degenerate=False)
+        stack = np.stack(frames_pos)
+        ab_all = np.concatenate([absorbers[f] + f * n for f in range(frames)])
+        finder.enumerate_many(stack, float(box[0]), ab_all, ipot)        # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kept, _ = finder.enumerate_many(stack, float(box[0]), ab_all, ipot)
+        torch.cuda.synchronize()
+        finder_s = time.perf_counter() - t0
+        lists = []
+        for nleg, v in sorted(kept.items()):
+            P = v["deg"].numel()
+            if P == 0:
+                continue
+            atoms = np.zeros((P, 3), np.int64)
+            atoms[:, :nleg - 1] = v["atoms"].cpu().numpy()
+            ipots = np.zeros((P, 4), np.int64)
+            ipots[:, :nleg] = v["ipots"].cpu().numpy()
+            ab = ab_all[v["slot"].cpu().numpy()]
+            lists.append(W.PathList(atoms, np.full(P, nleg), ab, ipots, v["deg"].double().cpu().numpy(), ab // n,
+                                    np.zeros(P)))
         pl = W.concat(lists, 4)
-        cfg = dict(workload="md-batch bcc 2000 atoms x 64 frames, 16 absorbers/frame, nleg 2-4, Rmax 5 A, k 0-18/0.05",
+        cfg = dict(workload="md-batch bcc 2000 atoms x 64 frames, 16 absorbers/frame, nleg 2-4, Rmax 5 A, "
+                            f"FEFF importance filter {critpw} %, k 0-18/0.05",
                    atoms=n, frames=frames, absorbers_per_frame=nabs, rmax_A=5.0, nleg_max=4, nk=len(q),
-                   phase="FEFF validation A018 V K edge (substitute for Fe K)", cell_A=float(box[0]))
-        return Workload(np.concatenate(pos_all), pl, q, float(box[0]), frames, cfg, [PHASE_K])
+                   critpw_percent=critpw, phase="FEFF validation A018 V K edge (substitute for Fe K)",
+                   cell_A=float(box[0]),
+                   path_finder=dict(ms_for_64_frames=finder_s * 1e3, paths_kept=len(pl),
+                                    note="one GPUPathFinder.enumerate_many call over the 64 frames (1024 absorbers): "
+                                         "candidates, enumeration, float32 importance and filter on the GPU, wall "
+                                         "clock including its host synchronisations"))
+        return Workload(np.concatenate(frames_pos), pl, q, float(box[0]), frames, cfg, [PHASE_K])
     if name == "fe":
         cl = W.cluster("bcc", 2.87, 9.0)
         pos = W.displaced(cl, 0.05, 7, keep_first=True)

@@ -43,21 +43,23 @@ __device__ __forceinline__ void wrap(const fm_find_sys &s, float (&v)[3]) {
     }
 }
 
-// The candidates of one absorber: atoms other than the absorber within rmax, in index order (block-wide scan).
+// The candidates of one absorber: atoms of its configuration (natom atoms from a multiple of natom: positions hold
+// configurations one after the other) other than the absorber within rmax, in index order (block-wide scan).
 __global__ __launch_bounds__(FD_THREADS) void k_find_cand(fm_find_sys s, const int32_t *absorbers, int32_t maxc,
                                                           int32_t *cidx, float *crel, float *cd0, int32_t *ncand) {
     __shared__ int wsum[FD_THREADS / 64];
     __shared__ int base_count;
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int a = absorbers[b];
+    const int first = (a / s.natom) * s.natom, last = first + s.natom;
     const float ax = s.pos[3 * a], ay = s.pos[3 * a + 1], az = s.pos[3 * a + 2];
     if (tid == 0) base_count = 0;
     __syncthreads();
-    for (int c0 = 0; c0 < s.natom; c0 += FD_THREADS) {
+    for (int c0 = first; c0 < last; c0 += FD_THREADS) {
         const int i = c0 + tid;
         bool keep = false;
         float v[3] = {0.f, 0.f, 0.f}, d = 0.f;
-        if (i < s.natom) {
+        if (i < last) {
             v[0] = s.pos[3 * i] - ax;
             v[1] = s.pos[3 * i + 1] - ay;
             v[2] = s.pos[3 * i + 2] - az;

@@ -171,7 +171,9 @@ class GPUPathFinder:
         return t, s
 
     def enumerate_many(self, pos, box, absorbers, ipot):
-        """The paths of every absorber in ``absorbers`` (A,) of one configuration pos (n, 3), in one pass.
+        """The paths of every absorber in ``absorbers`` (A,) in one pass: pos (n, 3) one configuration, or (C, n, 3)
+        configurations of the same atoms (frames of a trajectory), whose atoms count c n + i; every absorber's paths
+        run over the atoms of its own configuration. ipot (n,) is the potential of every atom of a configuration.
 
         Returns ({nleg: dict(coords, ipots, deg, xport, atoms, slot)}, ref (A,) float32): slot is the position of a
         path's absorber in ``absorbers``; the order is by absorber, then as ``enumerate`` orders the paths of one."""
@@ -180,15 +182,19 @@ class GPUPathFinder:
         st = N.stream_of(dev)
         if np.size(absorbers) == 0:
             return {}, np.zeros(0, np.float32)
-        p32 = torch.as_tensor(pos, device=dev).to(torch.float32).reshape(-1, 3).contiguous()
-        n = p32.shape[0]
+        p32 = torch.as_tensor(pos, device=dev).to(torch.float32)
+        n = p32.shape[-2] if p32.dim() == 3 else p32.reshape(-1, 3).shape[0]
+        p32 = p32.reshape(-1, 3).contiguous()
+        nconf = p32.shape[0] // max(n, 1)
         ab = torch.as_tensor(absorbers, device=dev).to(torch.int32).reshape(-1).contiguous()
         A = ab.numel()
         ip = torch.as_tensor(ipot, device=dev).to(torch.int32).reshape(-1).contiguous()
         if ip.numel() != n:
             raise ValueError("ipot needs one potential per atom")
-        if A and (int(ab.min()) < 0 or int(ab.max()) >= n):
-            raise ValueError("absorber index outside the configuration")
+        if A and (int(ab.min()) < 0 or int(ab.max()) >= n * nconf):
+            raise ValueError("absorber index outside the configurations")
+        if nconf > 1:
+            ip = ip.repeat(nconf)
         sysd = N.FmFindSys()
         sysd.natom, sysd.pos = n, p32.data_ptr()
         kind, cell, cinv = _cell32(box)

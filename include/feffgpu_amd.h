@@ -237,8 +237,8 @@ int fm_pair_histogram_grad(int32_t nconf, int32_t natom, const double *pos, cons
  * The enumeration runs in two passes: counts per work item, an exclusive scan (fm_exclusive_scan), then the paths
  * written at the scanned offsets; feffgpu_amd/pathfinder.py drives it. */
 typedef struct fm_find_sys {
-    int32_t natom;
-    const float *pos;            /* [natom][3] A */
+    int32_t natom;               /* atoms of one configuration */
+    const float *pos;            /* [nconf][natom][3] A: configurations one after the other */
     int32_t cell_kind;           /* 0 cluster, 1 cubic (cell[0] the length), 2 general (rows the lattice vectors) */
     float cell[9];
     float cellinv[9];
@@ -267,7 +267,8 @@ typedef struct fm_find_partial { /* partial paths of the 5+-leg extension */
     int32_t *slot;               /* [P] */
 } fm_find_partial;
 
-/* Candidates of each absorber (atoms other than it within rmax, minimum image, in index order): cidx [nabs][maxc],
+/* Candidates of each absorber (atoms of its own configuration other than it within rmax, minimum image, in index
+ * order; absorbers and candidate indices count atoms over all configurations): cidx [nabs][maxc],
  * crel [nabs][maxc][3], cd0 [nabs][maxc], ncand [nabs] (a count above maxc means the list was cut: call again with a
  * larger maxc). The candidate selection of GPUPathFinder.enumerate (pathfinder.py:229-236). */
 int fm_find_candidates(const fm_find_sys *sys, int32_t nabs, const int32_t *absorbers, int32_t maxc, int32_t *cidx,

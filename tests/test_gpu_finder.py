@@ -7,7 +7,7 @@
   same kept paths, apart from paths whose degeneracy-weighted importance ties the threshold to float32 rounding.
 * Paths of up to eight legs of a compact cluster (the 5+-leg extension) equal to the oracle's.
 * Symmetry totals on an ideal simple cubic lattice (shells of 6, 8, 12; triangles of 24 and 48).
-* Many absorbers in one pass equal to the absorbers one by one.
+* Many absorbers in one pass, of one configuration or of several, equal to the absorbers one by one.
 """
 import numpy as np
 import pytest
@@ -136,3 +136,17 @@ def test_finder_symmetry_totals_and_many_absorbers():
         for n, v in one.items():
             sel = many[n]["slot"] == s
             assert torch.equal(many[n]["atoms"][sel], v[4]) and torch.equal(many[n]["deg"][sel], v[2])
+    # configurations one after the other: each absorber's paths run over its own configuration's atoms
+    pos2 = W.displaced(pos, 0.05, 9)
+    frames = np.stack([pos, pos2])
+    n = len(pos)
+    ab2 = [9, n + 9, n + 127, 55]
+    many, refs = g.enumerate_many(frames, box[0], ab2, ipot)
+    for s, a in enumerate(ab2):
+        one, r1 = g.enumerate(frames[a // n], box[0], a % n, ipot)
+        assert abs(refs[s] - r1) == 0
+        for nl, v in one.items():
+            sel = many[nl]["slot"] == s
+            assert torch.equal(many[nl]["atoms"][sel], v[4] + (a // n) * n)
+            assert torch.equal(many[nl]["deg"][sel], v[2]) and torch.equal(many[nl]["ipots"][sel], v[1])
+            assert torch.equal(many[nl]["coords"][sel], v[0])
