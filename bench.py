@@ -386,12 +386,36 @@ def main():
     # whose gradient dL/dchi_seg = 2 q^4 (chi - t) is formed on the GPU between the two kernels; the whole step
     # (bucket kernels, segment sum, loss gradient, adjoint kernels, per-atom sum) captured as one hipGraph
     from feffgpu_amd.graph import CapturedStep
+    from feffgpu_amd.tune import apply_buckets, set_all, tune_buckets
     target = torch.tensor(s64 * 0.97, device=dev)
     q4 = torch.tensor(q ** 4, device=dev)
 
     def loss_grad(seg):
         return 2.0 * q4 * (seg - target)
     stream = torch.cuda.current_stream(dev)
+
+    def step_median_ms():
+        st = CapturedStep(plan32, len(pos), loss_grad=loss_grad)
+        st.pos.copy_(pos_t)
+        med = timed_replays(torch, st, args.steps, args.warmup, stream)[1]
+        del st
+        return med
+    # per-bucket launch policy (paths per workgroup, one workgroup per group or a persistent grid), and the whole
+    # step under each uniform policy beside the tuned one
+    buckets = tune_buckets(plan32, pos_t)
+    step_by_policy = {}
+    for name, grid, queue in (("per-group", 0, False), ("persistent", -1, False), ("persistent-queue", -1, True)):
+        set_all(plan32, grid, queue)
+        step_by_policy[name] = step_median_ms()
+    apply_buckets(plan32, buckets)
+    step_by_policy["tuned-per-bucket"] = step_median_ms()
+    print(f"[bench] step median ms by launch policy {step_by_policy}", file=sys.stderr, flush=True)
+    for r in buckets:
+        print(f"[bench] bucket kind {r['kind']} edge {r['edge']} nleg {r['nleg']} ({r['paths']} paths): forward "
+              f"{r['forward']['chosen']} {r['forward']['ms'][r['forward']['chosen']]:.3f} ms (default "
+              f"{r['forward']['ms']['budget/per-group']:.3f}), adjoint {r['adjoint']['chosen']} "
+              f"{r['adjoint']['ms'][r['adjoint']['chosen']]:.3f} ms (default "
+              f"{r['adjoint']['ms']['budget/per-group']:.3f})", file=sys.stderr, flush=True)
     step = CapturedStep(plan32, len(pos), loss_grad=loss_grad)
     step.pos.copy_(pos_t)
     ms_fb32, med_fb32, per = timed_replays(torch, step, args.steps, args.warmup, stream)
@@ -495,7 +519,20 @@ def main():
                ratio_fp64_fused_over_fp32_fused_fwd=ms_fwd64 / ms_fwd32,
                parity_vs_fp64=dict(r_factor_max=float(max(Rs)), max_rel=maxrel, grad_max_rel=grad_rel),
                launch_policy=dict(lds_budget_kib=policy["chosen"], forward_ms=policy["forward"],
-                                  backward_ms=policy["backward"]),
+                                  backward_ms=policy["backward"],
+                                  step_median_ms_by_policy=step_by_policy,
+                                  per_bucket=[dict(kind=r["kind"], edge=r["edge"], nleg=r["nleg"], paths=r["paths"],
+                                                   forward=r["forward"]["chosen"], adjoint=r["adjoint"]["chosen"],
+                                                   forward_ms={k: round(v, 4) for k, v in r["forward"]["ms"].items()},
+                                                   adjoint_ms={k: round(v, 4) for k, v in r["adjoint"]["ms"].items()})
+                                              for r in buckets],
+                                  note="per bucket: G paths per workgroup and grid (per-group = one workgroup per "
+                                       "group of G paths; persistent = as many workgroups as the CUs hold, looping "
+                                       "over the groups in a fixed order; persistent-queue = the same taking groups "
+                                       "from an atomic counter), the fastest of each bucket's launch timed alone; "
+                                       "step_median_ms_by_policy: the captured step with every bucket on one policy "
+                                       "(G by the LDS budget) and with the tuned per-bucket choice, which the "
+                                       "measured step uses"),
                roofline=roof, roofline_forward=roof_fwd, misfit=misfit, pair_tables=pairs)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)

@@ -73,6 +73,13 @@ class FmFindPartial(ctypes.Structure):
     _fields_ = [("seq", _vp), ("len", _vp), ("slot", _vp)]
 
 
+class FmLaunch(ctypes.Structure):
+    """The launch policy of one bucket: gpaths (0 by the LDS budget, else 1, 2, 4, 8), grid (0 one workgroup per
+    group, n > 0 a persistent grid of n, -1 as many as the CUs hold), tickets (a zeroed device uint32 pair: groups
+    taken from a counter; None: fixed assignment)."""
+    _fields_ = [("gpaths", _i32), ("grid", _i32), ("tickets", _vp)]
+
+
 def grid_runs(q, jq_rows, E_list):
     """qorder (q indices in ascending q) and qstart [nedge][stride]: for every edge, the first position in qorder
     whose lower interpolation index is >= e, for e = 0..E (the runs the adjoint of the fine grid gathers)."""
@@ -138,11 +145,12 @@ def lib():
                                            _vp, _vp, _vp]
         L.fm_chi_paths.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid), _i32,
                                    _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
-                                   ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, _vp]
+                                   ctypes.POINTER(_i32), _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(FmLaunch), _vp]
         L.fm_chi_paths_backward.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths),
                                             ctypes.POINTER(FmGrid), _i32, _i32, _vp, _i32, ctypes.POINTER(_i32),
                                             ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp, _vp,
-                                            _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]
+                                            _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
+                                            ctypes.POINTER(FmLaunch), _vp]
         L.fm_plain_path_bytes.restype = ctypes.c_size_t
         L.fm_plain_path_bytes.argtypes = [ctypes.POINTER(FmEdge), _i32]
         L.fm_chi_paths_plain.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid),

@@ -69,6 +69,7 @@ struct BArgs {
     size_t work_per_wave, work_off_F, work_off_acc, work_bytes;
     size_t dstride;             // fused adjoint: complex elements of a path's workspace region, (4 nleg + 1) E
     size_t dbytes;              // fused adjoint: bytes of the launch's regions (its paths and one spare)
+    unsigned *tickets;          // the path groups' counter pair of a persistent launch, or NULL (fixed assignment)
 };
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }
@@ -1087,7 +1088,7 @@ __device__ __forceinline__ double fine_grid_adjoint(int E, const int32_t *jq, co
 template <class T, int SET, int WPE>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_backward(BArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int pidx[8], pval[8];
+    __shared__ int pidx[8], pval[8], tslot;
     __shared__ double rbar[8];
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
     const int lane = lane_id();
@@ -1108,7 +1109,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
     unsigned char *const wg_work = a.work + (size_t)blockIdx.x * nband * a.work_per_wave;   // band b: + b * work_per_wave
     const size_t wpw = a.work_per_wave;
 
-    for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
+    for (int blk = first_group(a.tickets, tslot); blk * G < a.npath_bucket; blk = next_group(a.tickets, tslot, blk)) {
         // ---- 1. setup: one wavefront per path
         for (int j = wid; j < G; j += 4) {
             const int slot = blk * G + j;
@@ -1257,6 +1258,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
         }
         __syncthreads();
     }
+    release_groups(a.tickets);
 }
 
 // The fused sweep of a single-scattering path (fm_ss.h) at energy ee: cchi, and the complex derivatives of ptrac
@@ -1298,6 +1300,7 @@ __device__ __forceinline__ cx<T> ss_fused_energy(const fm_edge &ed, const PathLD
 template <class T, int SET, int NL, int WPE>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adjoint(BArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int tslot;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
     const int lane = lane_id();
     const fm_edge &ed = a.ed;
@@ -1313,7 +1316,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
     cx<T> *const wcc = reinterpret_cast<cx<T> *>(a.work);
     const int sbytes = E * (int)sizeof(cx<T>);
 
-    for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
+    for (int blk = first_group(a.tickets, tslot); blk * G < a.npath_bucket; blk = next_group(a.tickets, tslot, blk)) {
         {   // ---- 1. setup: the wavefront's paths j = wid, wid + 4 at once, a 16-lane group each (G <= 8)
             const int j = wid + 4 * (lane >> 4);
             const bool act = j < G;
@@ -1343,6 +1346,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
         }
         __syncthreads();
     }
+    release_groups(a.tickets);
 }
 
 // The per-path tail of the fused adjoint, one wavefront per path, from the workspace regions k_adjoint left:
@@ -1501,12 +1505,21 @@ __global__ __launch_bounds__(FM_WAVE *WPB_MAX) void k_chi_fine_bwd(fm_edge ed, i
 // Paths per workgroup: the largest plan within the LDS budget (by default one workgroup per CU, whose 4 wavefronts
 // then have the whole register file: 1 wavefront per SIMD); a plan small enough for two workgroups per CU runs the
 // variant sized for 2 wavefronts per SIMD (FP32).
-template <class T, int SET, int WPE> hipError_t launch_k(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
+// groups: the launch's path groups; grid, tickets: the bucket's policy; cap: the most workgroups the workspace holds.
+template <class T, int SET, int WPE>
+hipError_t launch_k(BArgs a, long groups, long cap, const fm_launch &pol, size_t shm, hipStream_t st) {
+    const void *fn = (const void *)k_backward<T, SET, WPE>;
     if (shm > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_backward<T, SET, WPE>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
+    long blocks = 0;
+    hipError_t e = launch_grid(fn, shm, groups, pol.grid, &blocks);
+    if (e != hipSuccess) return e;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) return hipSuccess;
+    // a grid capped below the groups by the workspace loops over them as a persistent grid does
+    a.tickets = pol.grid != 0 ? pol.tickets : nullptr;
     hipLaunchKernelGGL((k_backward<T, SET, WPE>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
@@ -1514,12 +1527,17 @@ template <class T, int SET, int WPE> hipError_t launch_k(const BArgs &a, long bl
 constexpr int FM_FUSED_MAXLEG = 6;   // multiple-scattering paths up to this many legs run the fused adjoint
 
 template <class T, int SET, int NL, int WPE = 1>
-hipError_t launch_fused_nl(const BArgs &a, long blocks, size_t shm, hipStream_t st) {
+hipError_t launch_fused_nl(BArgs a, long groups, const fm_launch &pol, size_t shm, hipStream_t st) {
     const void *fn = (const void *)k_adjoint<T, SET, NL, WPE>;
     if (shm > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
+    long blocks = 0;
+    hipError_t e = launch_grid(fn, shm, groups, pol.grid, &blocks);
+    if (e != hipSuccess) return e;
+    if (blocks == 0) return hipSuccess;
+    a.tickets = pol.grid != 0 ? pol.tickets : nullptr;
     hipLaunchKernelGGL((k_adjoint<T, SET, NL, WPE>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
@@ -1563,7 +1581,7 @@ constexpr size_t FM_DREGION_MAX = (size_t(1) << 31) - 4096;   // the sweep addre
 // The fused adjoint (multiple scattering of 3..FM_FUSED_MAXLEG legs, single scattering): k_adjoint's sweep, then
 // k_adjoint_tail, over chunks of the bucket of as many paths as the workspace holds regions for (one spare). Single
 // scattering of K edges keeps little in registers: two workgroups per CU (FP32).
-template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
+template <class T, int SET> hipError_t launch_fused(BArgs &a, const fm_launch &pol, hipStream_t st) {
     const fm_edge &ed = a.ed;
     const int nl = SET >= FM_SET_SS ? 2 : a.nlegmax;
     a.dstride = (size_t)(4 * nl + 1) * ed.E;
@@ -1575,8 +1593,8 @@ template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
     layout_sweep<T, SET>(A);
     size_t budget = fm_lds_budget(sizeof(T) == 4 ? 1 : 3);
     if (SET >= FM_SET_SS && budget > FM_LDS_SHARE) budget = FM_LDS_SHARE;
-    int g = pick_gpaths(ed, A.path_bytes, budget - 256);
-    if (g < 0) g = pick_gpaths(ed, A.path_bytes, FM_LDS_MAX - 256);
+    const int g = choose_gpaths(ed, A.path_bytes, budget, pol.gpaths);
+    if (g == -2) return hipErrorInvalidConfiguration;
     if (g < 0) return hipErrorInvalidValue;
     A.gp = 1 << g;
     A.bp = ed.bands[g];
@@ -1588,19 +1606,19 @@ template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
         A.first = B.first = a.first + (int)c0;
         A.npath_bucket = B.npath_bucket = cnt;
         A.dbytes = B.dbytes = (size_t)(cnt + 1) * per;
-        const long blocks = (cnt + A.gp - 1) / A.gp;
+        const long groups = (cnt + A.gp - 1) / A.gp;
         hipError_t e;
         if constexpr (SET >= FM_SET_SS) {
             // K: two workgroups per CU; L and M (27 and 21 representatives) would spill at that register budget
-            if (sizeof(T) == 4 && SET == FM_SET_SS && shm <= FM_LDS_SHARE - 256) e = launch_fused_nl<T, SET, 2, 2>(A, blocks, shm, st);
-            else e = launch_fused_nl<T, SET, 2, 1>(A, blocks, shm, st);
+            if (sizeof(T) == 4 && SET == FM_SET_SS && shm <= FM_LDS_SHARE - 256) e = launch_fused_nl<T, SET, 2, 2>(A, groups, pol, shm, st);
+            else e = launch_fused_nl<T, SET, 2, 1>(A, groups, pol, shm, st);
             if (e == hipSuccess) e = launch_tail<T, 2>(B, st);
         } else {
             switch (a.nlegmax) {
-                case 3: e = launch_fused_nl<T, SET, 3>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 3>(B, st); break;
-                case 4: e = launch_fused_nl<T, SET, 4>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 4>(B, st); break;
-                case 5: e = launch_fused_nl<T, SET, 5>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 5>(B, st); break;
-                case 6: e = launch_fused_nl<T, SET, 6>(A, blocks, shm, st); if (e == hipSuccess) e = launch_tail<T, 6>(B, st); break;
+                case 3: e = launch_fused_nl<T, SET, 3>(A, groups, pol, shm, st); if (e == hipSuccess) e = launch_tail<T, 3>(B, st); break;
+                case 4: e = launch_fused_nl<T, SET, 4>(A, groups, pol, shm, st); if (e == hipSuccess) e = launch_tail<T, 4>(B, st); break;
+                case 5: e = launch_fused_nl<T, SET, 5>(A, groups, pol, shm, st); if (e == hipSuccess) e = launch_tail<T, 5>(B, st); break;
+                case 6: e = launch_fused_nl<T, SET, 6>(A, groups, pol, shm, st); if (e == hipSuccess) e = launch_tail<T, 6>(B, st); break;
                 default: e = hipErrorInvalidValue;
             }
         }
@@ -1611,41 +1629,45 @@ template <class T, int SET> hipError_t launch_fused(BArgs &a, hipStream_t st) {
 
 // Multiple scattering of more than FM_FUSED_MAXLEG legs: pass A stores the prefix products in the workspace, pass B
 // reads them (k_backward). Everything else: the fused adjoint.
-template <class T, int SET> hipError_t launch_backward(BArgs &a, hipStream_t st) {
+template <class T, int SET> hipError_t launch_backward(BArgs &a, const fm_launch &pol, hipStream_t st) {
     const fm_edge &ed = a.ed;
     a.ltab = ed.L1 + 1;
     if constexpr (SET >= FM_SET_SS) {
-        return launch_fused<T, SET>(a, st);
+        return launch_fused<T, SET>(a, pol, st);
     } else {
-        if (a.nlegmax >= 3 && a.nlegmax <= FM_FUSED_MAXLEG) return launch_fused<T, SET>(a, st);
+        if (a.nlegmax >= 3 && a.nlegmax <= FM_FUSED_MAXLEG) return launch_fused<T, SET>(a, pol, st);
         layout<T, SET>(a, ed);
-        int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3) - 256);
-        if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+        const int g = choose_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 1 : 3), pol.gpaths);
+        if (g == -2) return hipErrorInvalidConfiguration;
         if (g < 0) return hipErrorInvalidValue;
         a.gp = 1 << g;
         a.bp = ed.bands[g];
         const size_t shm = a.path_bytes * a.gp;
         const long slots = (long)(a.work_bytes / a.work_per_wave);
         if (slots < a.bp.nband) return hipErrorInvalidValue;
-        long blocks = (a.npath_bucket + a.gp - 1) / a.gp;
-        if (blocks > slots / a.bp.nband) blocks = slots / a.bp.nband;
-        if (blocks == 0) return hipSuccess;
+        const long groups = (a.npath_bucket + a.gp - 1) / a.gp, cap = slots / a.bp.nband;
         if constexpr (sizeof(T) == 4) {
-            if (shm <= FM_LDS_SHARE - 256) return launch_k<T, SET, 2>(a, blocks, shm, st);
+            if (shm <= FM_LDS_SHARE - 256) return launch_k<T, SET, 2>(a, groups, cap, pol, shm, st);
         }
-        return launch_k<T, SET, 1>(a, blocks, shm, st);
+        return launch_k<T, SET, 1>(a, groups, cap, pol, shm, st);
     }
 }
 
-template <class T> hipError_t dispatch_b(BArgs &a, int set, hipStream_t st) {
+template <class T> hipError_t dispatch_b(BArgs &a, int set, const fm_launch &pol, hipStream_t st) {
     switch (set) {
-        case 0: return launch_backward<T, 0>(a, st);
-        case 1: return launch_backward<T, 1>(a, st);
-        case FM_SET_SS: return launch_backward<T, FM_SET_SS>(a, st);
-        case FM_SET_SS + 1: return launch_backward<T, FM_SET_SS + 1>(a, st);
-        case FM_SET_SS + 2: return launch_backward<T, FM_SET_SS + 2>(a, st);
+        case 0: return launch_backward<T, 0>(a, pol, st);
+        case 1: return launch_backward<T, 1>(a, pol, st);
+        case FM_SET_SS: return launch_backward<T, FM_SET_SS>(a, pol, st);
+        case FM_SET_SS + 1: return launch_backward<T, FM_SET_SS + 1>(a, pol, st);
+        case FM_SET_SS + 2: return launch_backward<T, FM_SET_SS + 2>(a, pol, st);
     }
     return hipErrorInvalidValue;
+}
+
+fm_launch policy_of_b(const fm_launch *launch, int b) {
+    fm_launch d;
+    memset(&d, 0, sizeof d);
+    return launch ? launch[b] : d;
 }
 
 int check_edge_b(const fm_edge *e) {
@@ -1728,7 +1750,8 @@ int fm_cchi_backward(const fm_edge *edge, int32_t precision, int32_t iorder, int
     a.work = (unsigned char *)workspace;
     a.work_bytes = workspace_bytes;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
+    const fm_launch pol = policy_of_b(nullptr, 0);
+    hipError_t e = precision == 32 ? dispatch_b<float>(a, set, pol, st) : dispatch_b<double>(a, set, pol, st);
     if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_cchi_backward: a path's tables do not fit one workgroup's LDS");
     if (e == hipErrorOutOfMemory) return set_err(FM_EINVAL, "fm_cchi_backward: workspace too small for one path");
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi_backward launch");
@@ -1792,7 +1815,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
                           const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          void *workspace, size_t workspace_bytes, void *stream) {
+                          void *workspace, size_t workspace_bytes, const fm_launch *launch, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     int emax = 0;
     for (int i = 0; i < nedge; ++i) {
@@ -1845,9 +1868,16 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         const int set = set_of(a.ed, a.nlegmax);
         a.work = (unsigned char *)workspace;
         a.work_bytes = workspace_bytes;
-        hipError_t e = precision == 32 ? dispatch_b<float>(a, set, st) : dispatch_b<double>(a, set, st);
+        const fm_launch pol = policy_of_b(launch, b);
+        if (pol.gpaths != 0 && pol.gpaths != 1 && pol.gpaths != 2 && pol.gpaths != 4 && pol.gpaths != 8)
+            return set_err(FM_EINVAL, "launch policy: gpaths must be 0, 1, 2, 4 or 8");
+        if (pol.grid < -1) return set_err(FM_EINVAL, "launch policy: grid must be -1, 0 or a number of workgroups");
+        hipError_t e = precision == 32 ? dispatch_b<float>(a, set, pol, st) : dispatch_b<double>(a, set, pol, st);
         if (e == hipErrorInvalidValue)
             return set_err(FM_EINVAL, "fm_chi_paths_backward: the bucket's paths do not fit one workgroup's LDS");
+        if (e == hipErrorInvalidConfiguration)
+            return set_err(FM_EINVAL, "fm_chi_paths_backward: the launch policy's gpaths has no band plan or does not "
+                                      "fit one CU's LDS");
         if (e == hipErrorOutOfMemory) return set_err(FM_EINVAL, "fm_chi_paths_backward: workspace too small for one path");
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths_backward launch");
     }

@@ -54,6 +54,7 @@ struct KArgs {
     int edge_id;          // index of the edge (rows of the grid's jq / fq)
     int nleg;             // the leg count of every path of this launch
     size_t path_bytes;    // LDS per path of a workgroup
+    unsigned *tickets;    // the path groups' counter pair of a persistent launch, or NULL (fixed assignment)
 };
 
 int set_err(int code, const char *msg) {
@@ -69,7 +70,7 @@ int hip_err(hipError_t e, const char *where) {
 template <class T, int SET>
 __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_WPE(T)))) void k_forward(KArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int pidx[8], pval[8];
+    __shared__ int pidx[8], pval[8], tslot;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / FM_WAVE);
     const int lane = lane_id();
     const fm_edge &ed = a.ed;
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
     unsigned char *lbase = smem + (size_t)jl * a.path_bytes;
     const PathLDS &gl = *reinterpret_cast<const PathLDS *>(lbase);
     const T *Wl = reinterpret_cast<const T *>(lbase + off_W);
-    for (int blk = blockIdx.x; blk * G < a.npath_bucket; blk += gridDim.x) {
+    for (int blk = first_group(a.tickets, tslot); blk * G < a.npath_bucket; blk = next_group(a.tickets, tslot, blk)) {
         // ---- setup. Single scattering (no tables): the wavefront's paths j = wid, wid + 4 at once, a 16-lane group
         // each (G <= 8); multiple scattering: one path after the other (measured faster there, the two workgroups of
         // a CU overlapping their setups)
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
         }
         __syncthreads();
     }
+    release_groups(a.tickets);
 }
 
 // GPUPathStage.chi_fine on its own: one wave per path reads cchi [npath][E] and runs the fine-grid epilogue.
@@ -187,15 +189,15 @@ int set_for(const fm_edge &ed, int nleg) {
     return ed.ilinit <= 1 ? 0 : 1;
 }
 
-// Paths per workgroup: the largest plan within the LDS budget (by default room for two workgroups per CU), else the
-// largest that fits one workgroup.
-template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st) {
+// Paths per workgroup: the policy's, else the largest plan within the LDS budget (by default room for two workgroups
+// per CU), else the largest that fits one workgroup. Grid: one workgroup per group, or the policy's persistent grid.
+template <class T, int SET> hipError_t launch_forward(KArgs &a, const fm_launch &pol, hipStream_t st) {
     const fm_edge &ed = a.ed;
     a.path_bytes = path_stride(align16(sizeof(PathLDS)) +
                                (SET < FM_SET_SS ? align16(sizeof(T) * rot_elems(a.nleg, ed.L1 + 1)) : 0) +
                                align16(sizeof(T) * ed.E) + align16(sizeof(double) * ed.E));
-    int g = pick_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 0 : 2) - 256);
-    if (g < 0) g = pick_gpaths(ed, a.path_bytes, FM_LDS_MAX - 256);
+    const int g = choose_gpaths(ed, a.path_bytes, fm_lds_budget(sizeof(T) == 4 ? 0 : 2), pol.gpaths);
+    if (g == -2) return hipErrorInvalidConfiguration;
     if (g < 0) return hipErrorInvalidValue;
     a.gp = 1 << g;
     a.bp = ed.bands[g];
@@ -205,21 +207,38 @@ template <class T, int SET> hipError_t launch_forward(KArgs &a, hipStream_t st) 
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const int blocks = (a.npath_bucket + a.gp - 1) / a.gp;
+    long blocks = 0;
+    hipError_t e = launch_grid(fn, shm, (a.npath_bucket + a.gp - 1) / a.gp, pol.grid, &blocks);
+    if (e != hipSuccess) return e;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_forward<T, SET>), dim3(blocks), dim3(FM_WAVE * 4), shm, st, a);
+    a.tickets = pol.grid != 0 ? pol.tickets : nullptr;
+    hipLaunchKernelGGL((k_forward<T, SET>), dim3((unsigned)blocks), dim3(FM_WAVE * 4), shm, st, a);
     return hipGetLastError();
 }
 
-template <class T> hipError_t dispatch(KArgs &a, int set, hipStream_t st) {
+template <class T> hipError_t dispatch(KArgs &a, int set, const fm_launch &pol, hipStream_t st) {
     switch (set) {
-        case 0: return launch_forward<T, 0>(a, st);
-        case 1: return launch_forward<T, 1>(a, st);
-        case FM_SET_SS: return launch_forward<T, FM_SET_SS>(a, st);
-        case FM_SET_SS + 1: return launch_forward<T, FM_SET_SS + 1>(a, st);
-        case FM_SET_SS + 2: return launch_forward<T, FM_SET_SS + 2>(a, st);
+        case 0: return launch_forward<T, 0>(a, pol, st);
+        case 1: return launch_forward<T, 1>(a, pol, st);
+        case FM_SET_SS: return launch_forward<T, FM_SET_SS>(a, pol, st);
+        case FM_SET_SS + 1: return launch_forward<T, FM_SET_SS + 1>(a, pol, st);
+        case FM_SET_SS + 2: return launch_forward<T, FM_SET_SS + 2>(a, pol, st);
     }
     return hipErrorInvalidValue;
+}
+
+// The policy of bucket b: the caller's, or the defaults (paths per workgroup by the LDS budget, one workgroup each).
+fm_launch policy_of(const fm_launch *launch, int b) {
+    fm_launch d;
+    memset(&d, 0, sizeof d);
+    return launch ? launch[b] : d;
+}
+
+const char *check_policy(const fm_launch &p) {
+    if (p.gpaths != 0 && p.gpaths != 1 && p.gpaths != 2 && p.gpaths != 4 && p.gpaths != 8)
+        return "launch policy: gpaths must be 0, 1, 2, 4 or 8";
+    if (p.grid < -1) return "launch policy: grid must be -1, 0 or a number of workgroups";
+    return nullptr;
 }
 
 int check_edge(const fm_edge *e) {
@@ -268,7 +287,8 @@ int fm_cchi(const fm_edge *edge, int32_t precision, int32_t iorder, int32_t npat
     a.edge_id = 0;
     a.nleg = nleg;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = precision == 32 ? dispatch<float>(a, set, st) : dispatch<double>(a, set, st);
+    const fm_launch pol = policy_of(nullptr, 0);
+    hipError_t e = precision == 32 ? dispatch<float>(a, set, pol, st) : dispatch<double>(a, set, pol, st);
     if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_cchi: a path's tables do not fit one workgroup's LDS");
     return e == hipSuccess ? FM_OK : hip_err(e, "fm_cchi launch");
 }
@@ -300,7 +320,7 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
                          int32_t precision, int32_t iorder, const int32_t *order, int32_t nbucket,
                          const int32_t *bucket_offset, const int32_t *bucket_set, const int32_t *bucket_edge,
                          const int32_t *bucket_nleg, int32_t nseg, const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg,
-                         void *chi_path, double *reff, void *stream) {
+                         void *chi_path, double *reff, const fm_launch *launch, void *stream) {
     if (nedge < 1 || nedge > FM_MAX_EDGE) return set_err(FM_EINVAL, "nedge out of range 1..4");
     for (int i = 0; i < nedge; ++i) {
         int rc = check_edge(&edges[i]);
@@ -339,8 +359,12 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         if ((kind == FM_BUCKET_K_MS) != (a.ed.ilinit <= 1) && kind != FM_BUCKET_SS)
             return set_err(FM_EINVAL, "bucket kind does not match its edge");
         const int set = set_for(a.ed, a.nleg);
-        hipError_t e = precision == 32 ? dispatch<float>(a, set, st) : dispatch<double>(a, set, st);
+        const fm_launch pol = policy_of(launch, b);
+        if (const char *m = check_policy(pol)) return set_err(FM_EINVAL, m);
+        hipError_t e = precision == 32 ? dispatch<float>(a, set, pol, st) : dispatch<double>(a, set, pol, st);
         if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_chi_paths: the bucket's paths do not fit one workgroup's LDS");
+        if (e == hipErrorInvalidConfiguration)
+            return set_err(FM_EINVAL, "fm_chi_paths: the launch policy's gpaths has no band plan or does not fit one CU's LDS");
         if (e != hipSuccess) return hip_err(e, "fm_chi_paths launch");
     }
     if (chi_seg && nseg > 0) {

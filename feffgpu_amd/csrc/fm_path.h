@@ -91,6 +91,65 @@ inline int pick_gpaths(const fm_edge &e, size_t path_bytes, size_t budget) {
     return -1;
 }
 
+// log2 of the paths per workgroup of a launch: `want` (1, 2, 4 or 8) if it has a band plan and fits one CU's LDS
+// (-2 if not), else (want 0) the largest within `budget`, else the largest that fits one CU (-1 if none).
+inline int choose_gpaths(const fm_edge &e, size_t path_bytes, size_t budget, int want) {
+    if (want > 0) {
+        const int g = want == 1 ? 0 : want == 2 ? 1 : want == 4 ? 2 : want == 8 ? 3 : -1;
+        if (g < 0 || e.bands[g].nband == 0 || (path_bytes << g) > FM_LDS_MAX - 256) return -2;
+        return g;
+    }
+    const int g = pick_gpaths(e, path_bytes, budget - 256);
+    return g >= 0 ? g : pick_gpaths(e, path_bytes, FM_LDS_MAX - 256);
+}
+
+// Workgroups of a launch over `groups` path groups: one per group (grid 0), or a persistent grid of `grid`
+// workgroups (-1: as many as the device's CUs hold at once with this kernel's registers and `shm` bytes of LDS),
+// never more than the groups. The kernel's dynamic LDS limit must already allow `shm`.
+inline hipError_t launch_grid(const void *fn, size_t shm, long groups, int grid, long *out) {
+    *out = groups;
+    if (grid == 0 || groups == 0) return hipSuccess;
+    long g = grid;
+    if (grid < 0) {
+        int dev = 0, cus = 0, per = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, FM_WAVE * 4, shm);
+        if (e != hipSuccess) return e;
+        g = (long)(cus > 0 ? cus : 1) * (per > 0 ? per : 1);
+    }
+    if (g < groups) *out = g;
+    return hipSuccess;
+}
+
+// ------------------------------------------------------------------------------------------------ path groups
+// Which group of paths a workgroup runs next. tickets NULL: workgroup b takes groups b, b + grid, b + 2 grid, ...
+// Otherwise every group comes from the counter tickets[0], one atomic per group; tickets[1] counts the workgroups
+// that have run out of groups, and the last of them puts both back to zero, so the pair (zero before the first
+// launch) is ready for the next launch that uses it on the same stream, with no fill in between. Every thread of the
+// workgroup calls these (they hold barriers when tickets is set); `slot` is a __shared__ int of the kernel.
+__device__ inline int take_group(unsigned *tickets, int &slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) slot = (int)atomicAdd(tickets, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(slot);
+}
+__device__ inline int first_group(unsigned *tickets, int &slot) {
+    return tickets ? take_group(tickets, slot) : (int)blockIdx.x;
+}
+__device__ inline int next_group(unsigned *tickets, int &slot, int blk) {
+    return tickets ? take_group(tickets, slot) : blk + (int)gridDim.x;
+}
+__device__ inline void release_groups(unsigned *tickets) {
+    if (tickets && threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(tickets + 1, 1u) == gridDim.x - 1) {
+            atomicExch(tickets, 0u);
+            atomicExch(tickets + 1, 0u);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------ setup of one path
 // Point j of path p of a list, in bohr relative to its absorber (point 0 = the absorber itself), by the minimum
 // image of the list's cell (src/feffgpu/exafs/forward.py:153-156, src/feffgpu/cell.py:26-35).

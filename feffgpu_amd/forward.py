@@ -128,21 +128,17 @@ class PathSpectra:
         self.C, self.n, self.A = C, n, absorbers.shape[1]
         ip = torch.ones(n, dtype=torch.long, device=self.device) if ipot is None else \
             torch.as_tensor(ipot, device=self.device)
-        parts = {}
-        for c in range(C):
-            kept, _ = self.finder.enumerate_many(pos[c], self.box, absorbers[c], ip)
-            for nleg, v in kept.items():
-                if v["deg"].numel() == 0:
-                    continue
-                g = parts.setdefault(nleg, ([], [], [], [], []))
-                ab = torch.as_tensor(absorbers[c], device=self.device, dtype=torch.long)
-                g[0].append(v["atoms"] + c * n)
-                g[1].append(ab[v["slot"].long()] + c * n)
-                g[2].append(v["ipots"])
-                g[3].append(v["deg"])
-                g[4].append(torch.full((v["deg"].numel(),), c, device=self.device, dtype=torch.long))
-        self.lists = {nleg: dict(atoms=torch.cat(g[0]), absorber=torch.cat(g[1]), ipots=torch.cat(g[2]),
-                                 deg=torch.cat(g[3]), copy=torch.cat(g[4])) for nleg, g in sorted(parts.items())}
+        # every copy's absorbers in one pass of the finder, numbered over the copies (c n + i)
+        ab = (torch.as_tensor(absorbers, device=self.device, dtype=torch.long)
+              + n * torch.arange(C, device=self.device)[:, None]).reshape(-1)
+        kept, _ = self.finder.enumerate_many(pos, self.box, ab, ip)
+        self.lists = {}
+        for nleg, v in sorted(kept.items()):
+            if v["deg"].numel() == 0:
+                continue
+            a = ab[v["slot"].long()]
+            self.lists[nleg] = dict(atoms=v["atoms"].long(), absorber=a, ipots=v["ipots"], deg=v["deg"],
+                                    copy=a // n)
         self.counts = {nleg: int(v["atoms"].shape[0]) for nleg, v in self.lists.items()}
         self._plans = {}
         return self.counts
@@ -224,20 +220,11 @@ class MultipleScattering:
         absorbers = torch.arange(n, device=self.device) if absorbers is None else \
             torch.as_tensor(absorbers, device=self.device, dtype=torch.long)
         A = int(absorbers.shape[0])
-        parts = {}
-        for c in range(C):
-            kept, _ = self.full.finder.enumerate_many(pos[c], self.box, absorbers, ipot)
-            for nleg in sorted(kept):
-                v = kept[nleg]
-                if nleg < 3 or v["deg"].numel() == 0:
-                    continue
-                g = parts.setdefault(nleg, ([], [], [], []))
-                g[0].append(v["atoms"] + c * n)
-                g[1].append(absorbers[v["slot"].long()] + c * n)
-                g[2].append(v["ipots"])
-                g[3].append(v["deg"])
-        self.lists = {nleg: dict(atoms=torch.cat(g[0]), absorber=torch.cat(g[1]), ipots=torch.cat(g[2]),
-                                 deg=torch.cat(g[3])) for nleg, g in sorted(parts.items())}
+        # every copy's absorbers in one pass of the finder, numbered over the copies (c n + i)
+        ab = (absorbers[None, :] + n * torch.arange(C, device=self.device)[:, None]).reshape(-1)
+        kept, _ = self.full.finder.enumerate_many(pos, self.box, ab, ipot)
+        self.lists = {nleg: dict(atoms=v["atoms"].long(), absorber=ab[v["slot"].long()], ipots=v["ipots"],
+                                 deg=v["deg"]) for nleg, v in sorted(kept.items()) if nleg >= 3 and v["deg"].numel()}
         self.C, self.n, self.A, self.n_abs = C, n, A, C * A
         self.abs_atoms = (absorbers[None, :] + n * torch.arange(C, device=self.device)[:, None]).reshape(-1)
         self.row_of = torch.full((C * n,), -1, device=self.device, dtype=torch.long)

@@ -30,6 +30,8 @@ from .pad import BOHR, load_phase, read_phase_pad
 
 NBETA = 40
 NLEG_MAX = 9
+# absorbers enumerated together: the four-leg counts take nc (nc + 1) int32 per absorber (nc candidates within rmax)
+ABSORBERS_PER_PASS = 4096
 PARTIAL_LIMIT = 30_000_000     # partial paths held per absorber (on average over a call) while 5+-leg paths grow
 _SEQ = N.FM_MAX_LEGS - 1
 
@@ -176,7 +178,25 @@ class GPUPathFinder:
         run over the atoms of its own configuration. ipot (n,) is the potential of every atom of a configuration.
 
         Returns ({nleg: dict(coords, ipots, deg, xport, atoms, slot)}, ref (A,) float32): slot is the position of a
-        path's absorber in ``absorbers``; the order is by absorber, then as ``enumerate`` orders the paths of one."""
+        path's absorber in ``absorbers``; the order is by absorber, then as ``enumerate`` orders the paths of one.
+        Absorbers are taken ABSORBERS_PER_PASS at a time (bounding the work arrays), the results joined."""
+        ab = torch.as_tensor(absorbers).reshape(-1)
+        if ab.numel() <= ABSORBERS_PER_PASS:
+            return self._enumerate_pass(pos, box, ab, ipot)
+        parts, refs = [], []
+        for s0 in range(0, ab.numel(), ABSORBERS_PER_PASS):
+            kept, ref = self._enumerate_pass(pos, box, ab[s0:s0 + ABSORBERS_PER_PASS], ipot)
+            for v in kept.values():
+                v["slot"] = v["slot"] + s0
+            parts.append(kept)
+            refs.append(ref)
+        out = {}
+        for nleg in sorted({k for kept in parts for k in kept}):
+            rows = [kept[nleg] for kept in parts if nleg in kept]
+            out[nleg] = {k: torch.cat([r[k] for r in rows]) for k in rows[0]}
+        return out, np.concatenate(refs)
+
+    def _enumerate_pass(self, pos, box, absorbers, ipot):
         L = N.lib()
         dev = self.device
         st = N.stream_of(dev)

@@ -17,6 +17,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -211,6 +213,42 @@ class PathPlan:
         self._h_kind = (N._i32 * max(1, len(self.buckets.kind)))(*self.buckets.kind.tolist())
         self._h_edge = (N._i32 * max(1, len(self.buckets.edge)))(*self.buckets.edge.tolist())
         self._h_nlm = (N._i32 * max(1, len(self.buckets.nleg)))(*self.buckets.nleg.tolist())
+        # per-bucket launch policies of the forward (0) and adjoint (1) calls: defaults until set_launch / tune;
+        # every bucket and kernel has its own zeroed counter pair for persistent launches that take groups in turn
+        nb = max(1, len(self.buckets.kind))
+        self._launch = [(N.FmLaunch * nb)(), (N.FmLaunch * nb)()]
+        self._tickets = torch.zeros(2, nb, 2, dtype=torch.int32, device=dev)
+        self._launch_set = False
+
+    def set_launch(self, kernel, bucket, gpaths=0, grid=0, queue=False):
+        """The launch policy of one bucket for the forward (kernel 0) or adjoint (kernel 1) calls: gpaths paths per
+        workgroup (0: by the LDS budget; else 1, 2, 4 or 8), grid (0: one workgroup per group of paths; n > 0: a
+        persistent grid of at most n workgroups; -1: as many as the CUs hold at once), queue (persistent grids take
+        the groups from an atomic counter, else in a fixed order)."""
+        if kernel not in (0, 1) or not 0 <= bucket < len(self.buckets.kind):
+            raise ValueError("kernel must be 0 or 1 and bucket one of the plan's")
+        if gpaths not in (0, 1, 2, 4, 8) or grid < -1:
+            raise ValueError("gpaths must be 0, 1, 2, 4 or 8 and grid -1, 0 or a number of workgroups")
+        L = self._launch[kernel][bucket]
+        L.gpaths, L.grid = int(gpaths), int(grid)
+        L.tickets = self._tickets[kernel, bucket].data_ptr() if (queue and grid != 0) else None
+        self._launch_set = True
+
+    def launch_policy(self):
+        """[{kind, edge, nleg, paths, forward: (gpaths, grid, queue), adjoint: (...)}] per bucket."""
+        out = []
+        for b in range(len(self.buckets.kind)):
+            pol = [(int(L[b].gpaths), int(L[b].grid), bool(L[b].tickets)) for L in self._launch]
+            out.append(dict(kind=int(self.buckets.kind[b]), edge=int(self.buckets.edge[b]),
+                            nleg=int(self.buckets.nleg[b]),
+                            paths=int(self.buckets.offset[b + 1] - self.buckets.offset[b]),
+                            forward=pol[0], adjoint=pol[1]))
+        return out
+
+    def _policy(self, kernel, b=None):
+        if b is not None:
+            return ctypes.pointer(self._launch[kernel][b])
+        return self._launch[kernel] if self._launch_set else None
 
     def _paths_struct(self, pos64):
         s = N.FmPaths()
@@ -240,7 +278,7 @@ class PathPlan:
                                          self.order_t.data_ptr(), len(self.buckets.kind), self._h_off, self._h_kind,
                                          self._h_edge, self._h_nlm, self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(),
                                          N.ptr(chi_seg), self.chi_path.data_ptr(), self.reff.data_ptr(),
-                                         N.stream_of(self.device)), "fm_chi_paths")
+                                         self._policy(0), N.stream_of(self.device)), "fm_chi_paths")
         elif want_seg:
             chi_seg.zero_()
         return chi_seg, self.chi_path, self.reff
@@ -267,7 +305,7 @@ class PathPlan:
                                               self._h_kind, self._h_edge, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
                                               N.ptr(gp), self.natom_ref, self.atom_ptr_t.data_ptr(),
                                               self.atom_refs_t.data_ptr(), self.gcoord.data_ptr(), grad.data_ptr(),
-                                              self.workspace.data_ptr(), self.workspace.numel(),
+                                              self.workspace.data_ptr(), self.workspace.numel(), self._policy(1),
                                               N.stream_of(self.device)), "fm_chi_paths_backward")
         return grad
 
@@ -314,8 +352,8 @@ class PathPlan:
         ps = self._paths_struct(pos64)
         N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
                                      self.order_t.data_ptr(), 1, off, kind, edge, nlm, 0, None, None, None,
-                                     self.chi_path.data_ptr(), self.reff.data_ptr(), N.stream_of(self.device)),
-                "fm_chi_paths")
+                                     self.chi_path.data_ptr(), self.reff.data_ptr(), self._policy(0, b),
+                                     N.stream_of(self.device)), "fm_chi_paths")
         return self.chi_path
 
     def backward_bucket(self, pos, gseg, b):
@@ -331,7 +369,7 @@ class PathPlan:
                                               self.iorder, self.order_t.data_ptr(), 1, off, kind, edge, nlm,
                                               self.segment_t.data_ptr(), gs.data_ptr(), None, 0, None, None,
                                               self.gcoord.data_ptr(), None, self.workspace.data_ptr(), self.workspace.numel(),
-                                              N.stream_of(self.device)), "fm_chi_paths_backward")
+                                              self._policy(1, b), N.stream_of(self.device)), "fm_chi_paths_backward")
         return self.gcoord
 
 

@@ -154,11 +154,28 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
  * chi_seg (double) and reff (double, A) may be NULL. Nothing is kept for the adjoint, which recomputes the
  * amplitudes. */
 enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_SS = 2 };
+
+/* Launch policy of one bucket (host array launch[nbucket], or NULL for the defaults; no counterpart in the reference,
+ * whose stage has no launch configuration). gpaths: paths per workgroup, 1, 2, 4 or 8 (refused where the edge has no
+ * band plan for it or its paths do not fit one CU's LDS), or 0: the largest within the LDS budget of
+ * fm_set_lds_budget. grid: 0 launches one workgroup per group of gpaths paths; n > 0 a persistent grid of at most n
+ * workgroups, -1 as many as the device's CUs hold at once, that loop over the groups. tickets (device, two uint32,
+ * zero before first use; used only with grid != 0): the groups are then taken from this counter one at a time, and
+ * the launch's last workgroup puts the pair back to zero, so it can serve every later launch on the same stream (not
+ * two launches that overlap in time); NULL: workgroup i takes groups i, i + grid, ... The multiple-scattering
+ * adjoint of 3-6 legs (and single scattering) applies the policy to its sweep; its per-path tail runs one wavefront
+ * per path. */
+typedef struct fm_launch {
+    int32_t gpaths;
+    int32_t grid;
+    uint32_t *tickets;
+} fm_launch;
+
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
                  int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,
                  const int32_t *bucket_kind, const int32_t *bucket_edge, const int32_t *bucket_nleg, int32_t nseg,
                  const int32_t *seg_ptr, const int32_t *seg_paths, double *chi_seg, void *chi_path, double *reff,
-                 void *stream);
+                 const fm_launch *launch, void *stream);
 
 /* Adjoint of fm_chi_paths: the same path list, buckets and grid (grid.qorder/qstart required); gseg [nseg][nk] =
  * dL/dchi_seg (double) with segment [npath] the row of each path, gpath [npath][nk] = dL/dchi_path (element type by
@@ -172,7 +189,7 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
                           const int32_t *bucket_offset, const int32_t *bucket_kind, const int32_t *bucket_edge,
                           const int32_t *bucket_nleg, const int32_t *segment, const double *gseg, const void *gpath, int32_t natom,
                           const int32_t *atom_ptr, const int32_t *atom_refs, double *gcoord, double *grad_pos,
-                          void *workspace, size_t workspace_bytes, void *stream);
+                          void *workspace, size_t workspace_bytes, const fm_launch *launch, void *stream);
 
 /* The plain FP64 path stage: the reference's op sequence (src/feffgpu/stage.py:147-388) with one kernel per stage
  * and every intermediate (rotation matrices, spherical-wave factors, gam / gamtl, scattering matrices, termination,

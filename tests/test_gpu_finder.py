@@ -110,7 +110,7 @@ def test_finder_long_paths_of_a_compact_cluster():
     print(f"[finder] compact cluster: {({n: len(v[4]) for n, v in ko.items()})}", flush=True)
 
 
-def test_finder_symmetry_totals_and_many_absorbers():
+def test_finder_symmetry_totals_and_many_absorbers(monkeypatch):
     from feffgpu_amd.pathfinder import GPUPathFinder
     a, m = 2.823, 3
     cells = np.stack(np.meshgrid(*[np.arange(-m, m + 1)] * 3, indexing="ij"), -1).reshape(-1, 3) * a
@@ -150,3 +150,12 @@ def test_finder_symmetry_totals_and_many_absorbers():
             assert torch.equal(many[nl]["atoms"][sel], v[4] + (a // n) * n)
             assert torch.equal(many[nl]["deg"][sel], v[2]) and torch.equal(many[nl]["ipots"][sel], v[1])
             assert torch.equal(many[nl]["coords"][sel], v[0])
+    # absorbers taken a few at a time: the same result as one pass
+    import feffgpu_amd.pathfinder as PF
+    monkeypatch.setattr(PF, "ABSORBERS_PER_PASS", 3)
+    split, rs = g.enumerate_many(frames, box[0], ab2, ipot)
+    np.testing.assert_array_equal(rs, refs)
+    assert set(split) == set(many)
+    for nl in many:
+        for k in many[nl]:
+            assert torch.equal(split[nl][k], many[nl][k]), (nl, k)

@@ -101,7 +101,7 @@ def test_bucket_leg_count_mismatch_gives_nan_rows():
     ps = plan._paths_struct(p)
     N.check(N.lib().fm_chi_paths(plan.edge_array, 1, ps, plan.grid, 32, 2, plan.order_t.data_ptr(),
                                  len(plan.buckets.kind), plan._h_off, plan._h_kind, plan._h_edge, h_nlm, 0, None, None,
-                                 None, plan.chi_path.data_ptr(), plan.reff.data_ptr(), N.stream_of(plan.device)),
+                                 None, plan.chi_path.data_ptr(), plan.reff.data_ptr(), None, N.stream_of(plan.device)),
             "fm_chi_paths")
     got = plan.chi_path.cpu()
     sel = plan.buckets.order[plan.buckets.offset[b4[0]]:plan.buckets.offset[b4[0] + 1]]

@@ -71,3 +71,69 @@ def test_every_lds_budget_against_oracle(case, npot, kib):
     err32 = float(np.abs(grads[32] - grads[64]).max() / np.abs(grads[64]).max())
     print(f"[policy] {case} {kib} KiB: FP32 vs FP64 gradient {err32:.2e}", flush=True)
     assert err32 < 1e-4
+
+
+@pytest.mark.parametrize("case,npot", [("A018", 2), ("A014", 1)], ids=["K", "L3"])
+def test_per_bucket_paths_per_workgroup_and_persistent_grids(case, npot):
+    """Every bucket launched with G = 1, 2, 4, 8 paths per workgroup (where the edge's band plan and one CU's LDS
+    allow it) and each grid: one workgroup per group, a persistent grid as large as the CUs hold and a small one of
+    3 workgroups, in a fixed order or taking the groups from a counter. At one G the persistent grids give the
+    per-group launch's chi and gradient bit for bit; every G agrees with the oracle (chi) and with the default
+    policy (gradient). Calls in a row and hipGraph replays check that the counters are back at zero after each
+    launch (a counter left behind would skip groups)."""
+    from feffgpu_amd.graph import CapturedStep
+    from feffgpu_amd.paths import PathPlan
+    phase, pos, box, pl = _list(case, npot)
+    ph = oracle.OraclePhase(phase)
+    ref, _, _ = oracle.chi_paths(ph, pos, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, Q, segment=pl.segment,
+                                 nseg=2, cell=box[0])
+    plan = PathPlan(str(phase), Q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, segment=pl.segment, nseg=2,
+                    cell=box[0], precision=32, device="cuda")
+    p = torch.tensor(pos, device="cuda")
+    gseg = torch.tensor(np.random.default_rng(5).normal(size=ref.shape) * Q ** 2, device="cuda")
+    nb = len(plan.buckets.kind)
+
+    def run():
+        seg, path, _ = plan.forward(p)
+        return seg.clone(), path.clone(), plan.backward(p, gseg)
+    seg0, _, g0 = run()
+    gscale = float(g0.abs().max())
+    grids = [(0, False), (-1, False), (-1, True), (3, False), (3, True)]
+    tried = 0
+    for G in (1, 2, 4, 8):
+        base = None
+        for grid, queue in grids:
+            for b in range(nb):
+                for kernel in (0, 1):
+                    plan.set_launch(kernel, b, G, grid, queue)
+            try:
+                out = run()
+            except ValueError:      # this G has no band plan on the edge, or its paths do not fit one CU
+                break
+            again = run()
+            for x, y in zip(out, again):
+                assert torch.equal(x, y), (G, grid, queue)
+            if base is None:
+                base = out
+                err = float(np.abs(out[0].cpu().numpy() - ref).max() / np.abs(ref).max())
+                gerr = float((out[2] - g0).abs().max()) / gscale
+                print(f"[policy] {case} G={G}: chi max rel {err:.2e}, gradient vs default {gerr:.2e}", flush=True)
+                assert err < 1e-5 and gerr < 1e-5
+            else:
+                for x, y in zip(out, base):
+                    assert torch.equal(x, y), (G, grid, queue)
+            tried += 1
+    assert tried >= 10
+    assert int(plan._tickets.abs().sum()) == 0
+    # persistent grids with counters inside a graph: replays equal the eager step
+    for b in range(nb):
+        for kernel in (0, 1):
+            plan.set_launch(kernel, b, 0, -1, True)
+    eager_seg, _, eager_g = run()
+    step = CapturedStep(plan, len(pos))
+    for _ in range(3):
+        seg, g = step(p, gseg)
+        torch.cuda.synchronize()
+        assert torch.equal(seg, eager_seg) and torch.equal(g, eager_g)
+        run()
+    assert int(plan._tickets.abs().sum()) == 0
