@@ -279,15 +279,16 @@ def dominant_bucket(plan, pl):
     return int(np.argmax(work)), work
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed counter summary (profiles/<kernel>_md_batch.md, a line
-    'traffic_bytes_per_launch: N' written from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy's
-    readings), or None when there is none."""
-    f = ROOT / "profiles" / f"{kernel}_md_batch.md"
+def measured_traffic(summary: str, kernel: str):
+    """HBM bytes per launch of `kernel` (e.g. "k_adjoint<float,0,3>") from the committed counter summary
+    profiles/<summary>.md, a line 'traffic_bytes_per_launch <kernel>: N' written from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes scaled by a plain copy's readings; None when there is none."""
+    f = ROOT / "profiles" / f"{summary}.md"
     if not f.exists():
         return None
+    key = f"traffic_bytes_per_launch {kernel}:"
     for line in f.read_text().splitlines():
-        if line.strip().startswith("traffic_bytes_per_launch:"):
+        if line.strip().startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
@@ -312,6 +313,9 @@ def main():
     ap.add_argument("--workload", default="md")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--policy", choices=["tuned", "per-group", "persistent", "persistent-queue"], default="tuned",
+                    help="launch policy of every bucket: tuned per bucket (default), or one fixed policy with the "
+                         "paths per workgroup from the LDS budget (for profiles whose launches must all match)")
     ap.add_argument("--kernel-only", choices=["fwd", "bwd", "copy"], default=None,
                     help="run only the dominant bucket's forward or adjoint kernel (or a plain device copy of known "
                          "size, for calibrating counters) --steps times, for rocprofv3 --pmc passes")
@@ -357,6 +361,12 @@ def main():
         policy64 = tune(plan64, pos_t, budgets=(80, 160))
         print(f"[bench] FP64 launch policy {policy64['chosen']}", file=sys.stderr, flush=True)
     print(f"[bench] launch policy {policy}", file=sys.stderr, flush=True)
+    from feffgpu_amd.tune import apply_buckets, set_all, tune_buckets
+    fixed = {"per-group": (0, False), "persistent": (-1, False), "persistent-queue": (-1, True)}
+    if args.kernel_only and args.policy == "tuned":
+        raise SystemExit("--kernel-only needs a fixed --policy, so that every launch of the kernel is the same")
+    if args.policy != "tuned":
+        set_all(plan32, *fixed[args.policy])
     bdom, b_work = dominant_bucket(plan32, pl)
     gseg_one = torch.ones(nseg, len(q), dtype=torch.float64, device=dev)
     if args.kernel_only:
@@ -386,7 +396,6 @@ def main():
     # whose gradient dL/dchi_seg = 2 q^4 (chi - t) is formed on the GPU between the two kernels; the whole step
     # (bucket kernels, segment sum, loss gradient, adjoint kernels, per-atom sum) captured as one hipGraph
     from feffgpu_amd.graph import CapturedStep
-    from feffgpu_amd.tune import apply_buckets, set_all, tune_buckets
     target = torch.tensor(s64 * 0.97, device=dev)
     q4 = torch.tensor(q ** 4, device=dev)
 
@@ -402,13 +411,14 @@ def main():
         return med
     # per-bucket launch policy (paths per workgroup, one workgroup per group or a persistent grid), and the whole
     # step under each uniform policy beside the tuned one
-    buckets = tune_buckets(plan32, pos_t)
-    step_by_policy = {}
-    for name, grid, queue in (("per-group", 0, False), ("persistent", -1, False), ("persistent-queue", -1, True)):
-        set_all(plan32, grid, queue)
-        step_by_policy[name] = step_median_ms()
-    apply_buckets(plan32, buckets)
-    step_by_policy["tuned-per-bucket"] = step_median_ms()
+    buckets, step_by_policy = [], {}
+    if args.policy == "tuned":
+        buckets = tune_buckets(plan32, pos_t)
+        for name, (grid, queue) in fixed.items():
+            set_all(plan32, grid, queue)
+            step_by_policy[name] = step_median_ms()
+        apply_buckets(plan32, buckets)
+        step_by_policy["tuned-per-bucket"] = step_median_ms()
     print(f"[bench] step median ms by launch policy {step_by_policy}", file=sys.stderr, flush=True)
     for r in buckets:
         print(f"[bench] bucket kind {r['kind']} edge {r['edge']} nleg {r['nleg']} ({r['paths']} paths): forward "
@@ -468,10 +478,12 @@ def main():
     ach_f = b_work[bdom] / (t_fwd * 1e-3) / 1e12
     nl = int(plan32.buckets.nleg[bdom])
     kset = int(plan32.buckets.kind[bdom]) if nl > 2 else 3 + plan32.tables[int(plan32.buckets.edge[bdom])].ilinit - 1
-    roof = dict(bound="valu", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak,
-                traffic=measured_traffic("k_adjoint") if (kset, nl) == (0, 4) and args.workload == "md" else None,
-                traffic_source="profiles/k_adjoint_md_batch.md (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-"
-                               "calibrated)" if (kset, nl) == (0, 4) and args.workload == "md" else None,
+    summary = f"k_adjoint_{args.workload}_batch" if args.workload == "md" else f"k_adjoint_{args.workload}"
+    traffic = measured_traffic(summary, f"k_adjoint<float,{kset},{nl}>")
+    roof = dict(bound="valu", achieved=ach_b, peak=peak, unit="TFLOP/s", frac=ach_b / peak, traffic=traffic,
+                traffic_source=None if traffic is None else
+                f"profiles/{summary}.md (rocprofv3 FETCH_SIZE / WRITE_SIZE passes of the sweep and the tail, "
+                "copy-calibrated)",
                 kernel=f"k_adjoint<float,{kset},{nl},*> + k_adjoint_tail<float,{nl}> ({what})", launch_ms=t_bwd,
                 flops_per_launch=2 * b_work[bdom], note=note)
     roof_fwd = dict(achieved=ach_f, peak=peak, unit="TFLOP/s", frac=ach_f / peak,
@@ -526,6 +538,7 @@ def main():
                                                    forward_ms={k: round(v, 4) for k, v in r["forward"]["ms"].items()},
                                                    adjoint_ms={k: round(v, 4) for k, v in r["adjoint"]["ms"].items()})
                                               for r in buckets],
+                                  bucket_policy=args.policy,
                                   note="per bucket: G paths per workgroup and grid (per-group = one workgroup per "
                                        "group of G paths; persistent = as many workgroups as the CUs hold, looping "
                                        "over the groups in a fixed order; persistent-queue = the same taking groups "
