@@ -370,8 +370,10 @@ def main():
     bdom, b_work = dominant_bucket(plan32, pl)
     gseg_one = torch.ones(nseg, len(q), dtype=torch.float64, device=dev)
     if args.kernel_only:
-        run = (lambda: plan32.forward_bucket(pos_t, bdom)) if args.kernel_only == "fwd" else \
-              (lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
+        keep = plan32.keep_amplitudes
+        plan32.forward_bucket(pos_t, bdom, keep=keep)
+        run = (lambda: plan32.forward_bucket(pos_t, bdom, keep=keep)) if args.kernel_only == "fwd" else \
+              (lambda: plan32.backward_bucket(pos_t, gseg_one, bdom, kept=keep))
         for _ in range(args.steps):
             run()
         sync()
@@ -419,6 +421,11 @@ def main():
             step_by_policy[name] = step_median_ms()
         apply_buckets(plan32, buckets)
         step_by_policy["tuned-per-bucket"] = step_median_ms()
+        if plan32.keep_amplitudes:
+            # the same step with the adjoint recomputing the amplitudes (nothing kept from the forward)
+            plan32.keep_amplitudes = False
+            step_by_policy["tuned-per-bucket, amplitudes recomputed"] = step_median_ms()
+            plan32.keep_amplitudes = True
     print(f"[bench] step median ms by launch policy {step_by_policy}", file=sys.stderr, flush=True)
     for r in buckets:
         print(f"[bench] bucket kind {r['kind']} edge {r['edge']} nleg {r['nleg']} ({r['paths']} paths): forward "
@@ -469,11 +476,14 @@ def main():
     kind_names = {0: "K multiple scattering", 1: "L/M multiple scattering", 2: "single scattering"}
     what = f"{kind_names.get(int(plan32.buckets.kind[bdom]))}, {int(plan32.buckets.nleg[bdom])} legs, " \
            f"{int(plan32.buckets.offset[bdom + 1] - plan32.buckets.offset[bdom])} paths"
-    t_bwd = launch_ms(lambda: plan32.backward_bucket(pos_t, gseg_one, bdom))
-    t_fwd = launch_ms(lambda: plan32.forward_bucket(pos_t, bdom))
+    keep = plan32.keep_amplitudes       # as in the step: the forward keeps the scattering matrices, the adjoint reads
+    t_fwd = launch_ms(lambda: plan32.forward_bucket(pos_t, bdom, keep=keep))
+    t_bwd = launch_ms(lambda: plan32.backward_bucket(pos_t, gseg_one, bdom, kept=keep))
     peak = MI355X_PEAK_TFLOPS["fp32"]
     note = ("FP32 vector pipes (no MFMA); 157.3 is MI355X's FP32 peak with packed math, which these kernels are "
-            "compiled without (non-packed VALU ceiling 78.6)")
+            "compiled without (non-packed VALU ceiling 78.6). " +
+            ("The adjoint reads the scattering matrices its forward kept; its algorithmic work is counted as twice "
+             "the forward's either way" if keep else "The adjoint recomputes the amplitudes"))
     ach_b = 2 * b_work[bdom] / (t_bwd * 1e-3) / 1e12
     ach_f = b_work[bdom] / (t_fwd * 1e-3) / 1e12
     nl = int(plan32.buckets.nleg[bdom])
@@ -517,8 +527,9 @@ def main():
                warmup=args.warmup, ms_per_step=ms_fb32, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="fp32 (fp64 geometry and phase)", data="synthetic",
                config=dict(cfg, paths=len(pl), parallelism="1 GPU"),
-               mode="fused FP32 forward + adjoint (amplitudes recomputed in the adjoint; k^2-weighted L2 loss per "
-                    "frame -> dL/dpositions), one hipGraph per step",
+               mode="fused FP32 forward + adjoint (" + ("the forward keeps the scattering matrices of the 3-6-leg "
+                    "paths for the adjoint" if plan32.keep_amplitudes else "amplitudes recomputed in the adjoint") +
+                    "; k^2-weighted L2 loss per frame -> dL/dpositions), one hipGraph per step",
                timing=f"{args.steps} graph replays between device synchronisations (wall clock); median of the "
                       "per-replay HIP events beside it",
                median_event_ms_per_step=med_fb32, min_event_ms_per_step=float(min(per)),

@@ -76,8 +76,9 @@ class FmFindPartial(ctypes.Structure):
 class FmLaunch(ctypes.Structure):
     """The launch policy of one bucket: gpaths (0 by the LDS budget, else 1, 2, 4, 8), grid (0 one workgroup per
     group, n > 0 a persistent grid of n, -1 as many as the CUs hold), tickets (a zeroed device uint32 pair: groups
-    taken from a counter; None: fixed assignment)."""
-    _fields_ = [("gpaths", _i32), ("grid", _i32), ("tickets", _vp)]
+    taken from a counter; None: fixed assignment), keep / keep_bytes (the bucket's kept scattering matrices: written
+    by the forward call, read by the adjoint call; None: kept nothing, recomputed)."""
+    _fields_ = [("gpaths", _i32), ("grid", _i32), ("tickets", _vp), ("keep", _vp), ("keep_bytes", ctypes.c_size_t)]
 
 
 def grid_runs(q, jq_rows, E_list):
@@ -152,6 +153,8 @@ def lib():
                                             _vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
                                             ctypes.POINTER(FmLaunch), _vp]
         L.fm_plain_path_bytes.restype = ctypes.c_size_t
+        L.fm_keep_bytes.restype = ctypes.c_size_t
+        L.fm_keep_bytes.argtypes = [_i32, ctypes.POINTER(FmEdge), _i32, _i32]
         L.fm_plain_path_bytes.argtypes = [ctypes.POINTER(FmEdge), _i32]
         L.fm_chi_paths_plain.argtypes = [ctypes.POINTER(FmEdge), _i32, ctypes.POINTER(FmPaths), ctypes.POINTER(FmGrid),
                                          _i32, _vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32),

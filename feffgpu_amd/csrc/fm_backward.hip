@@ -70,6 +70,7 @@ struct BArgs {
     size_t dstride;             // fused adjoint: complex elements of a path's workspace region, (4 nleg + 1) E
     size_t dbytes;              // fused adjoint: bytes of the launch's regions (its paths and one spare)
     unsigned *tickets;          // the path groups' counter pair of a persistent launch, or NULL (fixed assignment)
+    const void *keep;           // fused adjoint: the launch's kept scattering matrices (its first path first), or NULL
 };
 
 int set_err(int code, const char *msg) { return fm_set_err(code, msg); }
@@ -787,16 +788,33 @@ __device__ __forceinline__ void fused_fblocks(const fm_edge &ed, const PathLDS &
     if constexpr (S < NL - 1) fused_fblocks<T, SET, NL, S + 1>(ed, g, W, ltab, ee, F);
 }
 
-// The fused sweep at energy ee: returns cchi and stores the complex derivatives of ptrac in ds.
+// The kept matrices of scatterers S..NL-1 from the lane's rows (the first scatterer's block is L x L0, the last's
+// L0 x L, the others L x L).
+template <class T, int SET, int NL, int S>
+__device__ __forceinline__ void keep_loads(const T *kp, int E, cx<T> (&F)[NL - 1][FM_NREP]) {
+    constexpr int L = lamdef(SET).L, L0 = lamdef(SET).L0;
+    const T *b = kp + (size_t)2 * E * keep_off<SET>(NL, S);
+    if constexpr (S == 1) keep_load<T, SET, L, L0>(b, E, F[S - 1]);
+    else if constexpr (S == NL - 1) keep_load<T, SET, L0, L>(b, E, F[S - 1]);
+    else keep_load<T, SET, L, L>(b, E, F[S - 1]);
+    if constexpr (S < NL - 1) keep_loads<T, SET, NL, S + 1>(kp, E, F);
+}
+
+// The fused sweep at energy ee: returns cchi and stores the complex derivatives of ptrac in ds. kp: the lane's
+// pointer into the scattering matrices the forward kept (read instead of rebuilt), or NULL.
 template <class T, int SET, int NL>
 __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab,
-                                              int ee, DSlots<T> ds) {
+                                              int ee, DSlots<T> ds, const T *kp) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
     const int E = ed.E;
     cx<T> F[NL - 1][FM_NREP];
-    fused_fblocks<T, SET, NL, 1>(ed, g, W, ltab, ee, F);
+    if (kp) {
+        keep_loads<T, SET, NL, 1>(kp, E, F);
+    } else {
+        fused_fblocks<T, SET, NL, 1>(ed, g, W, ltab, ee, F);
+    }
     // Pre_n (L0 x L0)
     cx<T> Pn[L0][L0];
     {
@@ -1340,8 +1358,15 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
             const size_t reg = on ? (size_t)sj : (size_t)a.npath_bucket;
             const DSlots<T> ds{drs, (uint32_t)((reg * a.dstride + E + e) * sizeof(cx<T>)), sbytes, NL};
             cx<T> c;
-            if constexpr (SET >= FM_SET_SS) c = ss_fused_energy<T, SET - FM_SET_SS + 1>(ed, gj, e, ds);
-            else c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds);
+            if constexpr (SET >= FM_SET_SS) {
+                c = ss_fused_energy<T, SET - FM_SET_SS + 1>(ed, gj, e, ds);
+            } else {
+                // this lane's path (a duplicate in the launch for lanes past its end) and energy in the kept rows
+                const T *kp = a.keep ? reinterpret_cast<const T *>(a.keep) +
+                                           ((size_t)(vj ? sj : blk * G) * keep_rows<SET>(NL) * E + e) * 2
+                                     : nullptr;
+                c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds, kp);
+            }
             if (on) wcc[(size_t)sj * a.dstride + e] = c;
         }
         __syncthreads();
@@ -1601,8 +1626,11 @@ template <class T, int SET> hipError_t launch_fused(BArgs &a, const fm_launch &p
     const size_t shm = A.path_bytes * A.gp;
     layout_tail<T>(B, ed);
     if (B.path_bytes * 4 > FM_LDS_MAX) return hipErrorInvalidValue;
+    // bytes per path of the kept matrices
+    const size_t keep_path = SET >= FM_SET_SS ? 0 : (size_t)ed.E * keep_rows<SET>(nl) * 2 * sizeof(T);
     for (long c0 = 0; c0 < a.npath_bucket; c0 += chunk) {
         const int cnt = (int)(a.npath_bucket - c0 < chunk ? a.npath_bucket - c0 : chunk);
+        A.keep = a.keep ? (const unsigned char *)a.keep + (size_t)c0 * keep_path : nullptr;
         A.first = B.first = a.first + (int)c0;
         A.npath_bucket = B.npath_bucket = cnt;
         A.dbytes = B.dbytes = (size_t)(cnt + 1) * per;
@@ -1872,6 +1900,11 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
         if (pol.gpaths != 0 && pol.gpaths != 1 && pol.gpaths != 2 && pol.gpaths != 4 && pol.gpaths != 8)
             return set_err(FM_EINVAL, "launch policy: gpaths must be 0, 1, 2, 4 or 8");
         if (pol.grid < -1) return set_err(FM_EINVAL, "launch policy: grid must be -1, 0 or a number of workgroups");
+        if (pol.keep && kind != FM_BUCKET_SS && a.nlegmax >= 3 && a.nlegmax <= 6) {
+            if (pol.keep_bytes < fm_keep_bytes(precision, &a.ed, a.nlegmax, cnt))
+                return set_err(FM_EINVAL, "fm_chi_paths_backward: keep_bytes smaller than fm_keep_bytes for the bucket");
+            a.keep = pol.keep;
+        }
         hipError_t e = precision == 32 ? dispatch_b<float>(a, set, pol, st) : dispatch_b<double>(a, set, pol, st);
         if (e == hipErrorInvalidValue)
             return set_err(FM_EINVAL, "fm_chi_paths_backward: the bucket's paths do not fit one workgroup's LDS");

@@ -55,6 +55,7 @@ struct KArgs {
     int nleg;             // the leg count of every path of this launch
     size_t path_bytes;    // LDS per path of a workgroup
     unsigned *tickets;    // the path groups' counter pair of a persistent launch, or NULL (fixed assignment)
+    void *keep;           // the bucket's kept scattering matrices (written), or NULL
 };
 
 int set_err(int code, const char *msg) {
@@ -120,8 +121,15 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
             const bool active = il < nb && pval[jl];
             const int e = e0 + (il < nb ? il : nb - 1);
             cx<T> c;
-            if constexpr (SET >= FM_SET_SS) c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gl, e);
-            else c = path_cchi<T, SET>(ed, gl, Wl, ltab, e, nl);
+            if constexpr (SET >= FM_SET_SS) {
+                c = ss_cchi<T, SET - FM_SET_SS + 1>(ed, gl, e);
+            } else {
+                // this lane's (path, energy) in the kept rows, written by active lanes only
+                T *kp = (a.keep && active)
+                            ? reinterpret_cast<T *>(a.keep) + ((size_t)(blk * G + jl) * keep_rows<SET>(nl) * E + e) * 2
+                            : nullptr;
+                c = path_cchi<T, SET>(ed, gl, Wl, ltab, e, nl, kp);
+            }
             if (active) {
                 if (a.src.mode == 0) {
                     T *o = reinterpret_cast<T *>(a.cchi_out) + ((size_t)pidx[jl] * E + e) * 2;
@@ -255,6 +263,12 @@ extern "C" {
 const char *fm_last_error(void) { return g_err; }
 const char *fm_version(void) { return "feffgpu_amd 0.1 (gfx950)"; }
 
+size_t fm_keep_bytes(int32_t precision, const fm_edge *edge, int32_t nleg, int32_t npath) {
+    if (!edge || nleg < 3 || nleg > 6 || npath <= 0 || (precision != 32 && precision != 64)) return 0;
+    const int rows = edge->ilinit <= 1 ? keep_rows<0>(nleg) : keep_rows<1>(nleg);
+    return (size_t)npath * (size_t)rows * (size_t)edge->E * 2 * (precision == 32 ? 4 : 8);
+}
+
 int32_t fm_set_lds_budget(int32_t kernel, int32_t kib) {
     if (kernel < 0 || kernel > 3 || kib < 16 || kib > 160) return -1;
     return (int32_t)(g_lds_budget[kernel].exchange((size_t)kib * 1024) / 1024);
@@ -361,6 +375,11 @@ int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, con
         const int set = set_for(a.ed, a.nleg);
         const fm_launch pol = policy_of(launch, b);
         if (const char *m = check_policy(pol)) return set_err(FM_EINVAL, m);
+        if (pol.keep && kind != FM_BUCKET_SS && a.nleg >= 3 && a.nleg <= 6) {
+            if (pol.keep_bytes < fm_keep_bytes(precision, &a.ed, a.nleg, cnt))
+                return set_err(FM_EINVAL, "fm_chi_paths: keep_bytes smaller than fm_keep_bytes for the bucket");
+            a.keep = pol.keep;
+        }
         hipError_t e = precision == 32 ? dispatch<float>(a, set, pol, st) : dispatch<double>(a, set, pol, st);
         if (e == hipErrorInvalidValue) return set_err(FM_EINVAL, "fm_chi_paths: the bucket's paths do not fit one workgroup's LDS");
         if (e == hipErrorInvalidConfiguration)

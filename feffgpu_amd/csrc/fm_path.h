@@ -499,8 +499,69 @@ __device__ __forceinline__ cx<T> cfac_at(const fm_edge &ed, const PathLDS &g, in
 
 // cchi at energy e of the path held in g (tables in W). All lanes of the wave run it (e may be out of range for
 // some; they compute garbage that the caller drops, which keeps the loops uniform).
+// Kept scattering matrices: [path][row][energy] complex T, energy fastest, so that the lanes of a band (consecutive
+// energies of one path) write and read consecutive elements of each row. The rows of a path hold the
+// representatives of each scatterer's block, scatterer 1 first: its block is L x L0, the last one's L0 x L, the
+// others L x L (only those representatives are ever used). A lane's pointer kp is at (its path, row 0, its
+// energy); row r of the lane is at kp + 2 r E.
+__host__ __device__ constexpr bool keep_used(int set, int nr, int nc, int k) {
+    return pairdef(set).ri[k] < nr && pairdef(set).rj[k] < nc;
+}
+__host__ __device__ constexpr int keep_count(int set, int nr, int nc) {
+    int c = 0;
+    for (int k = 0; k < FM_NREP; ++k) c += keep_used(set, nr, nc, k) ? 1 : 0;
+    return c;
+}
+// the representatives of an NR x NC block, in index order
+struct KeepList {
+    int n;
+    int k[FM_NREP];
+};
+__host__ __device__ constexpr KeepList keep_list(int set, int nr, int nc) {
+    KeepList L{};
+    for (int k = 0; k < FM_NREP; ++k)
+        if (keep_used(set, nr, nc, k)) L.k[L.n++] = k;
+    return L;
+}
+// rows before scatterer s's block for a path of n legs (run-time n and s; the block sizes are compile-time
+// constants), and the rows of a path
+template <int SET> __host__ __device__ inline int keep_off(int n, int s) {
+    constexpr int c1 = keep_count(SET, lamdef(SET).L, lamdef(SET).L0), cm = keep_count(SET, lamdef(SET).L, lamdef(SET).L);
+    return s == 1 ? 0 : c1 + (s - 2) * cm;
+}
+template <int SET> __host__ __device__ inline int keep_rows(int n) {
+    constexpr int cl = keep_count(SET, lamdef(SET).L0, lamdef(SET).L);
+    return keep_off<SET>(n, n - 1) + cl;
+}
+
+template <class T> using cx2 = T __attribute__((ext_vector_type(2)));
+// one scatterer's NR x NC block, its first row at b
+template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void keep_store(T *b, int E, const cx<T> (&Ft)[FM_NREP]) {
+    constexpr KeepList KL = keep_list(SET, NR, NC);
+#pragma unroll
+    for (int q = 0; q < KL.n; ++q) {
+        cx2<T> v;
+        v.x = Ft[KL.k[q]].r;
+        v.y = Ft[KL.k[q]].i;
+        *reinterpret_cast<cx2<T> *>(b + (size_t)q * 2 * E) = v;
+    }
+}
+template <class T, int SET, int NR, int NC>
+__device__ __forceinline__ void keep_load(const T *b, int E, cx<T> (&Ft)[FM_NREP]) {
+    constexpr KeepList KL = keep_list(SET, NR, NC);
+#pragma unroll
+    for (int k = 0; k < FM_NREP; ++k) Ft[k] = mk<T>(T(0), T(0));
+#pragma unroll
+    for (int q = 0; q < KL.n; ++q) {
+        const cx2<T> v = *reinterpret_cast<const cx2<T> *>(b + (size_t)q * 2 * E);
+        Ft[KL.k[q]] = mk<T>(v.x, v.y);
+    }
+}
+
+// keep: the lane's pointer into the kept scattering matrices (written after each is built), or NULL.
 template <class T, int SET>
-__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n) {
+__device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int ltab, int e, int n, T *keep = nullptr) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
@@ -596,6 +657,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(n - 1, ilx, tl, Wl);
             fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
+            if (keep) keep_store<T, SET, L0, L>(keep + (size_t)2 * E * keep_off<SET>(n, n - 1), E, Ft);
             row_phases<T, SET>(ph, g, n);
             chain(std::integral_constant<int, L0>{});
         }
@@ -605,6 +667,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(s, ilx, tl, Wl);
             fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
+            if (keep) keep_store<T, SET, L, L>(keep + (size_t)2 * E * keep_off<SET>(n, s), E, Ft);
             row_phases<T, SET>(ph, g, s + 1);
             chain(std::integral_constant<int, L>{});
         }
@@ -614,6 +677,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             const T *tl, *Wl;
             scat(1, ilx, tl, Wl);
             fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
+            if (keep) keep_store<T, SET, L, L0>(keep, E, Ft);
             row_phases<T, SET>(ph, g, 2);
 #pragma unroll
             for (int a = 0; a < L0; ++a)

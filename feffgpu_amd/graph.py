@@ -42,11 +42,12 @@ class CapturedStep:
             self.out = self._body()
 
     def _body(self):
-        seg, _, _ = self.plan.forward(self.pos)
+        keep = self.adjoint and self.plan.keep_amplitudes
+        seg, _, _ = self.plan.forward(self.pos, keep=keep)
         if not self.adjoint:
             return seg, None
         g = self.loss_grad(seg) if self.loss_grad is not None else self.gseg
-        return seg, self.plan.backward(self.pos, g)
+        return seg, self.plan.backward(self.pos, g, kept=keep)
 
     def __call__(self, pos=None, gseg=None):
         """Replays the step (after copying pos / gseg into the static buffers when given). Returns (chi_seg,

@@ -81,11 +81,16 @@ class PathPlan:
     sigma2 (P,) or a number (default none). cell: None, a cubic length or a 3 x 3 cell. precision 32 or 64.
     convention: "feff" (linear interpolation of amplitude and phase, FEFF's, as the reference's forward model uses)
     or "larch" (the cubic spline_matrix of every edge over all its phase-grid energies, a dense interpolation).
-    The adjoint recomputes the amplitudes: nothing is kept between forward and backward.
+    keep_amplitudes: the differentiable call plan(pos) and CapturedStep keep the scattering matrices of the 3-6-leg
+    multiple-scattering paths from the forward call (npath (nleg - 1) 20 E complex per bucket, allocated at the
+    first such call) and the adjoint at the same positions reads them instead of recomputing them; False: nothing
+    is kept between forward and backward and the adjoint recomputes the amplitudes, as the reference's checkpointed
+    autograd does. The explicit forward / backward calls keep and use them only when asked (keep=, kept=).
     """
 
     def __init__(self, phases, q, atoms, nleg, absorber, ipots, deg, segment=None, nseg=None, edge=None, sigma2=None,
-                 cell=None, precision=32, device="cuda", iorder=2, convention="feff"):
+                 cell=None, precision=32, device="cuda", iorder=2, convention="feff", keep_amplitudes=True):
+        self.keep_amplitudes = bool(keep_amplitudes)
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
             raise RuntimeError("PathPlan runs on a ROCm GPU only")
@@ -219,6 +224,10 @@ class PathPlan:
         self._launch = [(N.FmLaunch * nb)(), (N.FmLaunch * nb)()]
         self._tickets = torch.zeros(2, nb, 2, dtype=torch.int32, device=dev)
         self._launch_set = False
+        # kept scattering matrices per bucket (allocated at the first forward call that keeps them); the generation
+        # counts those calls, so that an adjoint can tell whether the matrices are still those of its forward
+        self._keep = None
+        self.kept_generation = 0
 
     def set_launch(self, kernel, bucket, gpaths=0, grid=0, queue=False):
         """The launch policy of one bucket for the forward (kernel 0) or adjoint (kernel 1) calls: gpaths paths per
@@ -245,10 +254,35 @@ class PathPlan:
                             forward=pol[0], adjoint=pol[1]))
         return out
 
-    def _policy(self, kernel, b=None):
+    def _keep_buffers(self):
+        """Per bucket: (tensor, bytes) of its kept matrices, or None for buckets that keep nothing."""
+        if self._keep is None:
+            L = N.lib()
+            bufs = []
+            for b in range(len(self.buckets.kind)):
+                cnt = int(self.buckets.offset[b + 1] - self.buckets.offset[b])
+                nb = 0
+                if int(self.buckets.kind[b]) != 2 and cnt > 0:
+                    nb = int(L.fm_keep_bytes(self.precision, self.edges[int(self.buckets.edge[b])].struct,
+                                             int(self.buckets.nleg[b]), cnt))
+                bufs.append((torch.empty(nb, dtype=torch.uint8, device=self.device), nb) if nb else None)
+            self._keep = bufs
+        return self._keep
+
+    def _policy(self, kernel, b=None, keep=False):
+        """The launch options of the call: every bucket's (or bucket b's alone), with the kept matrices attached when
+        `keep` (the forward writes them, the adjoint reads them) and detached otherwise."""
+        bufs = self._keep_buffers() if keep else None
+        rng = range(len(self.buckets.kind)) if b is None else [b]
+        for i in rng:
+            L = self._launch[kernel][i]
+            if bufs is not None and bufs[i] is not None:
+                L.keep, L.keep_bytes = bufs[i][0].data_ptr(), bufs[i][1]
+            else:
+                L.keep, L.keep_bytes = None, 0
         if b is not None:
             return ctypes.pointer(self._launch[kernel][b])
-        return self._launch[kernel] if self._launch_set else None
+        return self._launch[kernel]
 
     def _paths_struct(self, pos64):
         s = N.FmPaths()
@@ -263,9 +297,10 @@ class PathPlan:
             s.cellinv[i] = float(self.cellinv[i])
         return s
 
-    def forward(self, pos, want_seg=True):
+    def forward(self, pos, want_seg=True, keep=False):
         """pos (N, 3) A. Returns (chi_seg (nseg, NK) float64, chi_path (P, NK) in the plan's precision, reff (P,) A).
-        chi_path and reff are the plan's own buffers, overwritten by the next call."""
+        chi_path and reff are the plan's own buffers, overwritten by the next call. keep: also keep the scattering
+        matrices for an adjoint at the same positions (backward(..., kept=True)); advances kept_generation."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         if pos64.dim() != 2 or pos64.shape[1] != 3:
             raise ValueError("pos must be (N, 3)")
@@ -278,14 +313,17 @@ class PathPlan:
                                          self.order_t.data_ptr(), len(self.buckets.kind), self._h_off, self._h_kind,
                                          self._h_edge, self._h_nlm, self.nseg, self.seg_ptr_t.data_ptr(), self.seg_paths_t.data_ptr(),
                                          N.ptr(chi_seg), self.chi_path.data_ptr(), self.reff.data_ptr(),
-                                         self._policy(0), N.stream_of(self.device)), "fm_chi_paths")
+                                         self._policy(0, keep=keep), N.stream_of(self.device)), "fm_chi_paths")
+            if keep:
+                self.kept_generation += 1
         elif want_seg:
             chi_seg.zero_()
         return chi_seg, self.chi_path, self.reff
 
-    def backward(self, pos, gseg=None, gpath=None):
+    def backward(self, pos, gseg=None, gpath=None, kept=False):
         """dL/dpos (N, 3) float64 given gseg = dL/dchi_seg (nseg, NK) and/or gpath = dL/dchi_path (P, NK), at the
-        positions pos (the amplitudes are recomputed inside the kernel)."""
+        positions pos. The amplitudes are recomputed inside the kernel, or (kept) read from the matrices the last
+        forward(..., keep=True) kept, which the caller guarantees ran at these positions."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         if self.max_atom >= pos64.shape[0]:
             raise ValueError("pos has too few rows for the path list")
@@ -305,7 +343,8 @@ class PathPlan:
                                               self._h_kind, self._h_edge, self._h_nlm, self.segment_t.data_ptr(), N.ptr(gs),
                                               N.ptr(gp), self.natom_ref, self.atom_ptr_t.data_ptr(),
                                               self.atom_refs_t.data_ptr(), self.gcoord.data_ptr(), grad.data_ptr(),
-                                              self.workspace.data_ptr(), self.workspace.numel(), self._policy(1),
+                                              self.workspace.data_ptr(), self.workspace.numel(),
+                                              self._policy(1, keep=kept and self._keep is not None),
                                               N.stream_of(self.device)), "fm_chi_paths_backward")
         return grad
 
@@ -340,7 +379,7 @@ class PathPlan:
                                      N.stream_of(self.device)), "fm_chi_paths_plain")
         return chi_seg, chi_path, reff
 
-    def forward_bucket(self, pos, b):
+    def forward_bucket(self, pos, b, keep=False):
         """Only bucket b of the plan (no segment sum): one kernel launch, for timing the kernel variants."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         if self.max_atom >= pos64.shape[0]:
@@ -352,12 +391,15 @@ class PathPlan:
         ps = self._paths_struct(pos64)
         N.check(N.lib().fm_chi_paths(self.edge_array, len(self.edges), ps, self.grid, self.precision, self.iorder,
                                      self.order_t.data_ptr(), 1, off, kind, edge, nlm, 0, None, None, None,
-                                     self.chi_path.data_ptr(), self.reff.data_ptr(), self._policy(0, b),
+                                     self.chi_path.data_ptr(), self.reff.data_ptr(), self._policy(0, b, keep),
                                      N.stream_of(self.device)), "fm_chi_paths")
+        if keep:
+            self.kept_generation += 1
         return self.chi_path
 
-    def backward_bucket(self, pos, gseg, b):
-        """The adjoint of bucket b only (gcoord of its paths; no per-atom sum): one kernel launch, for timing."""
+    def backward_bucket(self, pos, gseg, b, kept=False):
+        """The adjoint of bucket b only (gcoord of its paths; no per-atom sum): one kernel launch, for timing; kept:
+        from the matrices forward_bucket(..., keep=True) kept at these positions."""
         pos64 = torch.as_tensor(pos).detach().to(device=self.device, dtype=torch.float64).contiguous()
         gs = torch.as_tensor(gseg).to(device=self.device, dtype=torch.float64).contiguous()
         off = (N._i32 * 2)(int(self.buckets.offset[b]), int(self.buckets.offset[b + 1]))
@@ -369,22 +411,26 @@ class PathPlan:
                                               self.iorder, self.order_t.data_ptr(), 1, off, kind, edge, nlm,
                                               self.segment_t.data_ptr(), gs.data_ptr(), None, 0, None, None,
                                               self.gcoord.data_ptr(), None, self.workspace.data_ptr(), self.workspace.numel(),
-                                              self._policy(1, b), N.stream_of(self.device)), "fm_chi_paths_backward")
+                                              self._policy(1, b, kept and self._keep is not None),
+                                              N.stream_of(self.device)), "fm_chi_paths_backward")
         return self.gcoord
 
 
 class _ChiPathsFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pos, plan):
-        seg, path, _ = plan.forward(pos)
+        seg, path, _ = plan.forward(pos, keep=plan.keep_amplitudes)
         ctx.plan = plan
+        ctx.generation = plan.kept_generation if plan.keep_amplitudes else None
         ctx.save_for_backward(pos)
         return seg, path.clone()
 
     @staticmethod
     def backward(ctx, gseg, gpath):
         (pos,) = ctx.saved_tensors
-        g = ctx.plan.backward(pos, gseg, gpath)
+        # the kept matrices are this forward's unless another keeping forward ran since
+        kept = ctx.generation is not None and ctx.generation == ctx.plan.kept_generation
+        g = ctx.plan.backward(pos, gseg, gpath, kept=kept)
         return g.to(pos.dtype), None
 
 

@@ -74,12 +74,13 @@ def tune_buckets(plan, pos, reps: int = 3, gpaths=GPATHS, grids=GRIDS) -> list:
     .../persistent-queue; "budget/per-group" is the default)."""
     pos = torch.as_tensor(pos).to(device=plan.device, dtype=torch.float64)
     gseg = torch.ones(plan.nseg, plan.nk, dtype=torch.float64, device=plan.device)
+    keep = plan.keep_amplitudes     # as the captured step runs them: the forward keeps, the adjoint reads
     out = []
     for b in range(len(plan.buckets.kind)):
         row = dict(kind=int(plan.buckets.kind[b]), edge=int(plan.buckets.edge[b]), nleg=int(plan.buckets.nleg[b]),
                    paths=int(plan.buckets.offset[b + 1] - plan.buckets.offset[b]))
-        for kernel, name, fn in ((0, "forward", lambda: plan.forward_bucket(pos, b)),
-                                 (1, "adjoint", lambda: plan.backward_bucket(pos, gseg, b))):
+        for kernel, name, fn in ((0, "forward", lambda: plan.forward_bucket(pos, b, keep=keep)),
+                                 (1, "adjoint", lambda: plan.backward_bucket(pos, gseg, b, kept=keep))):
             times, choice = {}, {}
             for g, grid, queue in [(0, 0, False)] + [(g, gr, qu) for g in gpaths for gr, qu in grids]:
                 plan.set_launch(kernel, b, g, grid, queue)

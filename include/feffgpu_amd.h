@@ -155,21 +155,35 @@ int fm_chi_fine_backward(const fm_edge *edge, int32_t precision, int32_t npath, 
  * amplitudes. */
 enum { FM_BUCKET_K_MS = 0, FM_BUCKET_LM_MS = 1, FM_BUCKET_SS = 2 };
 
-/* Launch policy of one bucket (host array launch[nbucket], or NULL for the defaults; no counterpart in the reference,
- * whose stage has no launch configuration). gpaths: paths per workgroup, 1, 2, 4 or 8 (refused where the edge has no
- * band plan for it or its paths do not fit one CU's LDS), or 0: the largest within the LDS budget of
+/* Launch options of one bucket (host array launch[nbucket], or NULL for the defaults; no counterpart in the
+ * reference, whose stage has no launch configuration). gpaths: paths per workgroup, 1, 2, 4 or 8 (refused where the
+ * edge has no band plan for it or its paths do not fit one CU's LDS), or 0: the largest within the LDS budget of
  * fm_set_lds_budget. grid: 0 launches one workgroup per group of gpaths paths; n > 0 a persistent grid of at most n
  * workgroups, -1 as many as the device's CUs hold at once, that loop over the groups. tickets (device, two uint32,
  * zero before first use; used only with grid != 0): the groups are then taken from this counter one at a time, and
  * the launch's last workgroup puts the pair back to zero, so it can serve every later launch on the same stream (not
  * two launches that overlap in time); NULL: workgroup i takes groups i, i + grid, ... The multiple-scattering
  * adjoint of 3-6 legs (and single scattering) applies the policy to its sweep; its per-path tail runs one wavefront
- * per path. */
+ * per path.
+ * keep (device, keep_bytes >= fm_keep_bytes(precision, edge, nleg, paths of the bucket)), for buckets of multiple
+ * scattering of 3-6 legs (ignored by the others): fm_chi_paths writes the scattering matrices of every path's
+ * scatterers into it ([path][row][energy] complex, paths in bucket order; the rows of a path are the independent
+ * entries of each scatterer's block, scatterer 1 first), and fm_chi_paths_backward reads them instead of
+ * recomputing them. The caller guarantees that the
+ * adjoint call sees the positions of the forward call that wrote the buffer; NULL: the forward keeps nothing and the
+ * adjoint recomputes the amplitudes, as the reference's checkpointed autograd does. */
 typedef struct fm_launch {
     int32_t gpaths;
     int32_t grid;
     uint32_t *tickets;
+    void *keep;
+    size_t keep_bytes;
 } fm_launch;
+
+/* Bytes of a bucket's kept scattering matrices: npath x rows x E complex elements of the precision, the rows being
+ * 14 (K edges: the 6 x 4 and 4 x 6 blocks of the first and last scatterers) or 20 per scatterer (0 for buckets that
+ * keep nothing: nleg outside 3..6). */
+size_t fm_keep_bytes(int32_t precision, const fm_edge *edge, int32_t nleg, int32_t npath);
 
 int fm_chi_paths(const fm_edge *edges, int32_t nedge, const fm_paths *paths, const fm_grid *grid, int32_t precision,
                  int32_t iorder, const int32_t *order, int32_t nbucket, const int32_t *bucket_offset,

@@ -1,5 +1,6 @@
 """The step captured as a hipGraph (feffgpu_amd.graph.CapturedStep): every replay equals the eager forward + adjoint
-at the same positions, bit for bit, with eager work on the same plan and host synchronisations between replays."""
+at the same positions, bit for bit, with eager work on the same plan and host synchronisations between replays. The
+eager reference runs as the step does: the forward keeps the scattering matrices, the adjoint reads them."""
 import numpy as np
 import pytest
 import torch
@@ -34,9 +35,9 @@ def test_captured_step_replays_equal_eager():
         return 2.0 * q4 * (seg - target)
 
     def eager(p):
-        seg, _, _ = plan.forward(p)
+        seg, _, _ = plan.forward(p, keep=True)
         seg = seg.clone()
-        return seg, plan.backward(p, loss_grad(seg)).clone()
+        return seg, plan.backward(p, loss_grad(seg), kept=True).clone()
 
     poss = [torch.tensor(W.displaced(pos0, 0.07, s), device="cuda") for s in (40, 41)]
     ref = [eager(p) for p in poss]
@@ -56,9 +57,9 @@ def test_captured_forward_only_and_static_gseg():
     from feffgpu_amd.graph import CapturedStep
     pos0, plan = _setup()
     p = torch.tensor(W.displaced(pos0, 0.07, 50), device="cuda")
-    seg_ref = plan.forward(p)[0].clone()
+    seg_ref = plan.forward(p, keep=True)[0].clone()
     gs = torch.tensor(np.random.default_rng(2).normal(size=(3, len(Q))), device="cuda")
-    g_ref = plan.backward(p, gs).clone()
+    g_ref = plan.backward(p, gs, kept=True).clone()
     fwd = CapturedStep(plan, len(pos0), adjoint=False)
     both = CapturedStep(plan, len(pos0))
     for _ in range(3):

@@ -108,8 +108,8 @@ def test_captured_step_with_r_factor_loss():
     win = FitWindow(3.0, 16.0, 1.0, 2, 1.0, 5.0)
     loss = RFactorLoss(target, q, win, scale=0.5)
     p = torch.tensor(pos, device="cuda")
-    seg = plan.forward(p)[0].clone()
-    g_eager = plan.backward(p, loss(seg).clone()).clone()
+    seg = plan.forward(p, keep=True)[0].clone()                         # as the captured step runs it
+    g_eager = plan.backward(p, loss(seg).clone(), kept=True).clone()
     rf_eager = loss.rfac.clone()
     torch.cuda.synchronize()
     step = CapturedStep(plan, len(pos0), loss_grad=loss)

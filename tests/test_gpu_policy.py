@@ -129,11 +129,14 @@ def test_per_bucket_paths_per_workgroup_and_persistent_grids(case, npot):
     for b in range(nb):
         for kernel in (0, 1):
             plan.set_launch(kernel, b, 0, -1, True)
-    eager_seg, _, eager_g = run()
+    def run_kept():           # as the captured step runs: the forward keeps the scattering matrices
+        seg, path, _ = plan.forward(p, keep=True)
+        return seg.clone(), path.clone(), plan.backward(p, gseg, kept=True)
+    eager_seg, _, eager_g = run_kept()
     step = CapturedStep(plan, len(pos))
     for _ in range(3):
         seg, g = step(p, gseg)
         torch.cuda.synchronize()
         assert torch.equal(seg, eager_seg) and torch.equal(g, eager_g)
-        run()
+        run_kept()
     assert int(plan._tickets.abs().sum()) == 0
