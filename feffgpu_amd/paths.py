@@ -82,15 +82,18 @@ class PathPlan:
     convention: "feff" (linear interpolation of amplitude and phase, FEFF's, as the reference's forward model uses)
     or "larch" (the cubic spline_matrix of every edge over all its phase-grid energies, a dense interpolation).
     keep_amplitudes: the differentiable call plan(pos) and CapturedStep keep the scattering matrices of the 3-6-leg
-    multiple-scattering paths from the forward call (npath (nleg - 1) 20 E complex per bucket, allocated at the
-    first such call) and the adjoint at the same positions reads them instead of recomputing them; False: nothing
-    is kept between forward and backward and the adjoint recomputes the amplitudes, as the reference's checkpointed
-    autograd does. The explicit forward / backward calls keep and use them only when asked (keep=, kept=).
+    multiple-scattering paths from the forward call (fm_keep_bytes per bucket: about 28-88 complex per path and
+    energy, allocated at the first such call) and the adjoint at the same positions reads them instead of
+    recomputing them; False: nothing is kept between forward and backward and the adjoint recomputes the amplitudes,
+    as the reference's checkpointed autograd does. A plan whose buffers would exceed keep_limit_bytes keeps nothing.
+    The explicit forward / backward calls keep and use them only when asked (keep=, kept=).
     """
 
     def __init__(self, phases, q, atoms, nleg, absorber, ipots, deg, segment=None, nseg=None, edge=None, sigma2=None,
-                 cell=None, precision=32, device="cuda", iorder=2, convention="feff", keep_amplitudes=True):
+                 cell=None, precision=32, device="cuda", iorder=2, convention="feff", keep_amplitudes=True,
+                 keep_limit_bytes=16 << 30):
         self.keep_amplitudes = bool(keep_amplitudes)
+        self.keep_limit_bytes = int(keep_limit_bytes)
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
             raise RuntimeError("PathPlan runs on a ROCm GPU only")
@@ -218,6 +221,8 @@ class PathPlan:
         self._h_kind = (N._i32 * max(1, len(self.buckets.kind)))(*self.buckets.kind.tolist())
         self._h_edge = (N._i32 * max(1, len(self.buckets.edge)))(*self.buckets.edge.tolist())
         self._h_nlm = (N._i32 * max(1, len(self.buckets.nleg)))(*self.buckets.nleg.tolist())
+        if self.keep_amplitudes and self.keep_bytes() > self.keep_limit_bytes:
+            self.keep_amplitudes = False
         # per-bucket launch policies of the forward (0) and adjoint (1) calls: defaults until set_launch / tune;
         # every bucket and kernel has its own zeroed counter pair for persistent launches that take groups in turn
         nb = max(1, len(self.buckets.kind))
@@ -254,19 +259,27 @@ class PathPlan:
                             forward=pol[0], adjoint=pol[1]))
         return out
 
+    def _keep_sizes(self):
+        L = N.lib()
+        out = []
+        for b in range(len(self.buckets.kind)):
+            cnt = int(self.buckets.offset[b + 1] - self.buckets.offset[b])
+            nb = 0
+            if int(self.buckets.kind[b]) != 2 and cnt > 0:
+                nb = int(L.fm_keep_bytes(self.precision, self.edges[int(self.buckets.edge[b])].struct,
+                                         int(self.buckets.nleg[b]), cnt))
+            out.append(nb)
+        return out
+
+    def keep_bytes(self) -> int:
+        """Device bytes the kept scattering matrices of this plan take."""
+        return sum(self._keep_sizes())
+
     def _keep_buffers(self):
         """Per bucket: (tensor, bytes) of its kept matrices, or None for buckets that keep nothing."""
         if self._keep is None:
-            L = N.lib()
-            bufs = []
-            for b in range(len(self.buckets.kind)):
-                cnt = int(self.buckets.offset[b + 1] - self.buckets.offset[b])
-                nb = 0
-                if int(self.buckets.kind[b]) != 2 and cnt > 0:
-                    nb = int(L.fm_keep_bytes(self.precision, self.edges[int(self.buckets.edge[b])].struct,
-                                             int(self.buckets.nleg[b]), cnt))
-                bufs.append((torch.empty(nb, dtype=torch.uint8, device=self.device), nb) if nb else None)
-            self._keep = bufs
+            self._keep = [(torch.empty(nb, dtype=torch.uint8, device=self.device), nb) if nb else None
+                          for nb in self._keep_sizes()]
         return self._keep
 
     def _policy(self, kernel, b=None, keep=False):

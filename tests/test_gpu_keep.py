@@ -138,3 +138,27 @@ def test_generation_and_graph_and_differences():
         assert torch.equal(seg, eager_seg)
         assert _close(g, plan.backward(torch.tensor(pos, device="cuda"), w), 64)
     print("[keep] generations, graph replays and differences", flush=True)
+
+
+def test_plan_over_its_keep_limit_recomputes():
+    """A plan whose kept matrices would exceed keep_limit_bytes keeps nothing: the differentiable call and a captured
+    step recompute the amplitudes in the adjoint, and give the recomputing adjoint's gradient."""
+    from feffgpu_amd.graph import CapturedStep
+    from feffgpu_amd.paths import PathPlan
+    phase, pos, box, pl = _box("A018", 2)
+    kw = dict(segment=pl.segment, nseg=4, cell=box, precision=32, device="cuda")
+    plan = PathPlan(str(phase), Q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, keep_limit_bytes=1, **kw)
+    need = plan.keep_bytes()
+    assert need > 1 and not plan.keep_amplitudes
+    w = torch.tensor(np.random.default_rng(4).normal(size=(4, len(Q))) * Q ** 2, device="cuda")
+    p = torch.tensor(pos, device="cuda", requires_grad=True)
+    seg, _ = plan(p)
+    (g,) = torch.autograd.grad((seg * w).sum(), p)
+    assert plan._keep is None and plan.kept_generation == 0
+    assert _close(g, plan.backward(p.detach(), w), 32)
+    step = CapturedStep(plan, len(pos))
+    _, g2 = step(pos, w)
+    torch.cuda.synchronize()
+    assert _close(g2, g, 32) and plan._keep is None
+    roomy = PathPlan(str(phase), Q, pl.atoms, pl.nleg, pl.absorber, pl.ipots, pl.deg, keep_limit_bytes=need, **kw)
+    assert roomy.keep_amplitudes
