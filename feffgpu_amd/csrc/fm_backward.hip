@@ -362,6 +362,7 @@ __device__ __forceinline__ cx<T> pass_A(const fm_edge &ed, const PathLDS &g, con
     const T *Wn = W + (size_t)(n - 1) * ltab * FM_NREP;
     const T *chw = TL<T>::chw(ed);
     cx<T> xa[3], yb[2];
+#pragma unroll
     for (int r = 1; r <= 4; ++r) {
         X.seed_row(r, xa, r <= lx1);
         if (r < 4) Y.seed_row(r, yb, r <= lxn); else Y.step(r, yb, r <= lxn);
@@ -515,6 +516,7 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
         const T *chw = TL<T>::chw(ed);
         T gX = T(0), gY = T(0), gB = T(0);
         cx<T> xa[3], xd[3], ya[2], yd[2];
+#pragma unroll
         for (int r = 1; r <= 4; ++r) {
             X.seed_row(r, xa, xd, r <= lx1);
             if (r < 4) Y.seed_row(r, ya, yd, r <= lxn); else Y.step(r, ya, yd, r <= lxn);
@@ -611,14 +613,19 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
     cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
     const size_t r2 = (size_t)(ilx < 2 ? ilx : 2) * tlstride;
     cx<T> tn = mk<T>(tl[r2], tl[r2 + 1]);
-    const int rseed = ilx < 4 ? ilx : 4;
-    for (int r = 1; r <= rseed; ++r) {
+    // rows 1..4 unrolled (their seeds are explicit per row); a lane whose energy has fewer rows runs them with the
+    // t-matrix factor zero, which adds nothing
+#pragma unroll
+    for (int r = 1; r <= 4; ++r) {
         X.seed_row(r, xa, xd);
         if (r < 4) Y.seed_row(r, ya, yd); else Y.step(r, ya, yd);
         const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
         const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
-        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, Wl + (size_t)r * FM_NREP,
-                                  Wdl + (size_t)r * FM_NREP, dX, dY, dB);
+        // rows past ilx read the table's row ilx (inside the table, finite) and add zero
+        const cx<T> tr = r <= ilx ? t : mk<T>(T(0), T(0));
+        const size_t rw = (size_t)(r <= ilx ? r : ilx) * FM_NREP;
+        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * tr, ya[1] * tr, yd[0] * tr, yd[1] * tr, Wl + rw, Wdl + rw, dX,
+                                  dY, dB);
         t = tn;
         tn = tnn;
     }
@@ -865,6 +872,7 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
         const T *chw = TL<T>::chw(ed);
         cx<T> dX = mk<T>(T(0), T(0)), dY = dX, dB = dX;
         cx<T> xa[3], xd[3], ya[2], yd[2];
+#pragma unroll
         for (int r = 1; r <= 4; ++r) {
             X.seed_row(r, xa, xd, r <= lx1);
             if (r < 4) Y.seed_row(r, ya, yd, r <= lxn); else Y.step(r, ya, yd, r <= lxn);

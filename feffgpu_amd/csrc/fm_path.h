@@ -411,13 +411,16 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
     cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
     const size_t r2 = (size_t)(ilx < 2 ? ilx : 2) * tlstride;
     cx<T> tn = mk<T>(tl[r2], tl[r2 + 1]);
-    const int rseed = ilx < 4 ? ilx : 4;
-    for (int r = 1; r <= rseed; ++r) {
+    // rows 1..4 unrolled (their seeds are explicit per row); a lane whose energy has fewer rows runs them with the
+    // t-matrix factor zero (and the table's row ilx, inside the table and finite), which adds nothing
+#pragma unroll
+    for (int r = 1; r <= 4; ++r) {
         X.seed_row(r, xa);
         if (r < 4) Y.seed_row(r, yb); else Y.step(r, yb);
         const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
         const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
-        accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
+        const cx<T> tr = r <= ilx ? t : mk<T>(T(0), T(0));
+        accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * tr, yb[1] * tr, Wl + (size_t)(r <= ilx ? r : ilx) * FM_NREP);
         t = tn;
         tn = tnn;
     }
@@ -587,6 +590,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
         const T *Wn = W + (size_t)(n - 1) * ltab * FM_NREP;
         const T *chw = TL<T>::chw(ed);
         cx<T> xa[3], yb[2];
+#pragma unroll
         for (int r = 1; r <= 4; ++r) {
             X.seed_row(r, xa, r <= lx1);
             if (r < 4) Y.seed_row(r, yb, r <= lxn); else Y.step(r, yb, r <= lxn);
