@@ -446,8 +446,8 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
         }
         if (r == ilx) row(r);
     } else {
-#pragma unroll 1
-        for (int r = 5; r <= ilx; ++r) {
+        // two rows per iteration as well (the state renamed rather than copied), the table values read where used
+        auto row = [&](int r) {
             X.step(r, xa);
             Y.step(r, yb);
             const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
@@ -455,7 +455,14 @@ __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T r
             accum_row<T, SET, NR, NC>(Ft, xa, yb[0] * t, yb[1] * t, Wl + (size_t)r * FM_NREP);
             t = tn;
             tn = tnn;
+        };
+        int r = 5;
+#pragma unroll 1
+        for (; r < ilx; r += 2) {
+            row(r);
+            row(r + 1);
         }
+        if (r == ilx) row(r);
     }
 }
 

@@ -145,6 +145,16 @@ template <class V, int C> struct HankRows {
 #pragma unroll
         for (int c = 0; c < C; ++c) { p2[c] = p1[c]; p1[c] = o[c]; }
     }
+    // row r > C + 1 (past every column's explicit rows): the recursion alone, the same arithmetic as row()
+    __device__ __forceinline__ void row_rec(int r, V (&o)[C], bool alive) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (c == 0) o[c] = (p2[0] - (z * p1[0]) * T(2 * r - 3)) * T(alive ? 1 : 0);
+            else o[c] = (p2[c] - (z * (p1[c] + p1[c - 1])) * T(2 * r - 3)) * T(alive ? 1 : 0);
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) { p2[c] = p1[c]; p1[c] = o[c]; }
+    }
 };
 
 // The representatives of T and F of a single-scattering path at energy ee (V = cx<T> or dcx<T> for the derivative
@@ -164,21 +174,30 @@ __device__ __forceinline__ void ss_reps(const fm_edge &ed, int ee, int ip_s, V z
     HankRows<V, C> H;
     H.init(z);
     const T *tl = tlbase + ((size_t)ip_s * ltab) * (2 * E) + 2 * ee;
-    const int rmax = ilx > 4 ? ilx : 4;
-#pragma unroll 1
-    for (int r = 1; r <= rmax; ++r) {
+    // rows 1 .. RP (the columns' explicit rows and the termination's) unrolled, so that their cases resolve at
+    // compile time; the rest (r <= ilx) by the recursion alone, two rows per iteration
+    constexpr int RP = C + 1 > 4 ? C + 1 : 4;
+    auto scatter_row = [&](int r, const V (&h)[C]) {
+        const T sgn = (r & 1) ? T(1) : T(-1);          // (-1)^(il - 1)
+        const cx<T> t = mk<T>(tl[(size_t)r * 2 * E], tl[(size_t)r * 2 * E + 1]);
+        V yt[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) yt[c] = (h[c] * t) * sgn;
+#pragma unroll
+        for (int k = 0; k < D.nrep; ++k)
+            if (r >= D.ilmin[k]) Fk[k] = Fk[k] + h[D.ca[k] - 1] * yt[D.cb[k] - 1];
+    };
+    auto rec_row = [&](int r) {
+        V h[C];
+        H.row_rec(r, h, r <= lxa);
+        scatter_row(r, h);
+    };
+#pragma unroll
+    for (int r = 1; r <= RP; ++r) {
         V h[C];
         H.row(r, h, r <= lxa);
         const T sgn = (r & 1) ? T(1) : T(-1);          // (-1)^(il - 1)
-        if (r <= ilx) {
-            const cx<T> t = mk<T>(tl[(size_t)r * 2 * E], tl[(size_t)r * 2 * E + 1]);
-            V yt[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) yt[c] = (h[c] * t) * sgn;
-#pragma unroll
-            for (int k = 0; k < D.nrep; ++k)
-                if (r >= D.ilmin[k]) Fk[k] = Fk[k] + h[D.ca[k] - 1] * yt[D.cb[k] - 1];
-        }
+        if (r <= ilx) scatter_row(r, h);
         if (r <= 4) {
             int ch = -1;
             for (int q = 0; q < ed.nchan; ++q)
@@ -194,6 +213,13 @@ __device__ __forceinline__ void ss_reps(const fm_edge &ed, int ee, int ip_s, V z
             }
         }
     }
+    int r = RP + 1;
+#pragma unroll 1
+    for (; r < ilx; r += 2) {
+        rec_row(r);
+        rec_row(r + 1);
+    }
+    if (r == ilx) rec_row(r);
 }
 
 // e^{-i eta m} for m = -3..3 of leg J
