@@ -334,7 +334,9 @@ __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path
             rmax = rows[q] > rmax ? rows[q] : rmax;
             dlm1[q] = dl[q] = tlm1[q] = tl[q] = 0.0;
         }
-        for (int il0 = 1; il0 <= rmax; il0 += BATCH) {
+        // rows in batches of BATCH (their coefficients fetched together); the seeds l <= 2 fall in the first batch,
+        // which is peeled so that the seed-or-recursion choice is made at compile time
+        auto batch = [&](int il0, auto first) {
             RotCoef<BATCH> rc[ILP];
 #pragma unroll
             for (int q = 0; q < ILP; ++q) rc[q].fetch(ed, il0, rows[q], mm[q], nn[q]);
@@ -345,7 +347,9 @@ __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path
                 for (int q = 0; q < ILP; ++q) {
                     if (il > rows[q]) continue;
                     double d, t1 = 0.0;
-                    if (l <= 2) {
+                    // the first batch starts at row 1 (l = u); batches of fewer than 3 rows test l at run time
+                    const bool seed = decltype(first)::value ? (u <= 2) : (BATCH < 3 && l <= 2);
+                    if (seed) {
                         d = dseed(l, m1[q], m2[q], c[q], sn[q], c2[q], s2[q]);
                         if constexpr (WITH_D) t1 = dseed_db(l, m1[q], m2[q], c[q], sn[q], c2[q], s2[q]);
                     } else {
@@ -361,7 +365,9 @@ __device__ void build_tables(const fm_edge &ed, unsigned char *smem, size_t path
                     }
                 }
             }
-        }
+        };
+        if (rmax >= 1) batch(1, std::true_type{});
+        for (int il0 = 1 + BATCH; il0 <= rmax; il0 += BATCH) batch(il0, std::false_type{});
     }
     wave_sync();
 }
