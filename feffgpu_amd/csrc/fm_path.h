@@ -402,7 +402,8 @@ __device__ __forceinline__ void accum_row(cx<T> (&Ft)[FM_NREP], const cx<T> (&xa
 // Accumulates the representatives k with ri < NR, rj < NC into Ft (zeroed here). Each lane runs its own rows
 // il = 1..ilx (the wave runs to the largest ilx of its lanes); the t-matrix is fetched two rows ahead.
 // PREFETCH: each row's table values are read into registers before its recursion steps (the adjoint's sweep, at one
-// wavefront per SIMD, has no second wavefront to hide the LDS latency); otherwise where the compiler puts them.
+// wavefront per SIMD, has no second wavefront to hide the LDS latency; the forward measures faster with it too);
+// otherwise where the compiler puts them.
 template <class T, int SET, int NR, int NC, bool PREFETCH = false>
 __device__ __forceinline__ void fblock(cx<T> (&Ft)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
                                        const T *__restrict__ Wl, const T *__restrict__ tl, int tlstride) {
@@ -673,7 +674,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             int ilx;
             const T *tl, *Wl;
             scat(n - 1, ilx, tl, Wl);
-            fblock<T, SET, L0, L>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
+            fblock<T, SET, L0, L, true>(Ft, ck, (T)g.r[n], (T)g.r[n - 1], ilx, Wl, tl, tlstride);
             if (keep) keep_store<T, SET, L0, L>(keep + (size_t)2 * E * keep_off<SET>(n, n - 1), E, Ft);
             row_phases<T, SET>(ph, g, n);
             chain(std::integral_constant<int, L0>{});
@@ -683,7 +684,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             int ilx;
             const T *tl, *Wl;
             scat(s, ilx, tl, Wl);
-            fblock<T, SET, L, L>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
+            fblock<T, SET, L, L, true>(Ft, ck, (T)g.r[s + 1], (T)g.r[s], ilx, Wl, tl, tlstride);
             if (keep) keep_store<T, SET, L, L>(keep + (size_t)2 * E * keep_off<SET>(n, s), E, Ft);
             row_phases<T, SET>(ph, g, s + 1);
             chain(std::integral_constant<int, L>{});
@@ -693,7 +694,7 @@ __device__ cx<T> path_cchi(const fm_edge &ed, const PathLDS &g, const T *W, int 
             int ilx;
             const T *tl, *Wl;
             scat(1, ilx, tl, Wl);
-            fblock<T, SET, L, L0>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
+            fblock<T, SET, L, L0, true>(Ft, ck, (T)g.r[2], (T)g.r[1], ilx, Wl, tl, tlstride);
             if (keep) keep_store<T, SET, L, L0>(keep, E, Ft);
             row_phases<T, SET>(ph, g, 2);
 #pragma unroll
