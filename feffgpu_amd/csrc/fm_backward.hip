@@ -568,19 +568,28 @@ __device__ __forceinline__ void pass_B(const fm_edge &ed, const PathLDS &g, cons
 // the termination with its tangents against Pre_n = F_{n-1} ... F_1, then s = n-1 .. 1: K_s from Pre_s (rebuilt
 // from the kept F) and U_s = T F_{n-1} ... F_{s+1}, the eta derivative, U_{s-1}, and the tangent stream of F_s.
 
-// One partial-wave row of a block's tangents contracted with K, accumulated as complex derivatives.
+// Bit 2 aa + bb set: some representative of an NR x NC block has Hankel columns (aa + 1, bb + 1).
+template <int SET, int NR, int NC>
+__host__ __device__ constexpr unsigned group_mask() {
+    constexpr PairDef PD = pairdef(SET);
+    unsigned m = 0;
+    for (int k = 0; k < FM_NREP; ++k)
+        if (PD.ri[k] < NR && PD.rj[k] < NC) m |= 1u << (2 * (PD.a[k] - 1) + PD.b[k] - 1);
+    return m;
+}
+
+// One partial-wave row of a block's tangents contracted with K, accumulated as complex derivatives. y, yd: the
+// gamtl leg's factors and their tangents without the row's t-matrix factor t, which scales the row's three sums once.
 template <class T, int SET, int NR, int NC, class WRow>
 __device__ __forceinline__ void bwd_row_c(const cx<T> (&K)[FM_NREP], const cx<T> (&xa)[3], const cx<T> (&xd)[3],
-                                          cx<T> y0, cx<T> y1, cx<T> y0d, cx<T> y1d, const WRow &w,
+                                          const cx<T> (&y)[2], const cx<T> (&yd)[2], cx<T> t, const WRow &w,
                                           const WRow &wd, cx<T> &dX, cx<T> &dY, cx<T> &dB) {
     constexpr PairDef PD = pairdef(SET);
-    cx<T> Z[3][2], Q[3][2], Qd[3][2];
+    constexpr unsigned used = group_mask<SET, NR, NC>();
+    const cx<T> zero = mk<T>(T(0), T(0));
+    cx<T> Q[3][2], Qd[3][2];
 #pragma unroll
-    for (int aa = 0; aa < 3; ++aa) {
-        Z[aa][0] = xa[aa] * y0;
-        Z[aa][1] = xa[aa] * y1;
-        Q[aa][0] = Q[aa][1] = Qd[aa][0] = Qd[aa][1] = mk<T>(T(0), T(0));
-    }
+    for (int aa = 0; aa < 3; ++aa) Q[aa][0] = Q[aa][1] = Qd[aa][0] = Qd[aa][1] = zero;
 #pragma unroll
     for (int k = 0; k < FM_NREP; ++k)
         if (PD.ri[k] < NR && PD.rj[k] < NC) {
@@ -588,14 +597,26 @@ __device__ __forceinline__ void bwd_row_c(const cx<T> (&K)[FM_NREP], const cx<T>
             rfma(Q[aa][bb], K[k], w[k]);
             rfma(Qd[aa][bb], K[k], wd[k]);
         }
+    // dB: sum Qd[aa][bb] xa[aa] y[bb]; dY: sum Q[aa][bb] xa[aa] yd[bb]; dX: sum Q[aa][bb] xd[aa] y[bb]
+    cx<T> sB = zero, sY = zero, sX = zero;
 #pragma unroll
-    for (int aa = 0; aa < 3; ++aa) {
-        cfma(dB, Qd[aa][0], Z[aa][0]);
-        cfma(dB, Qd[aa][1], Z[aa][1]);
-        cfma(dX, xd[aa], Q[aa][0] * y0 + Q[aa][1] * y1);
+    for (int bb = 0; bb < 2; ++bb) {
+        if (!((used >> bb) & 0x15u)) continue;
+        cx<T> rd = zero, r = zero, v = zero;
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa)
+            if ((used >> (2 * aa + bb)) & 1u) {
+                cfma4(rd, Qd[aa][bb], xa[aa]);
+                cfma4(r, Q[aa][bb], xa[aa]);
+                cfma4(v, Q[aa][bb], xd[aa]);
+            }
+        cfma4(sB, y[bb], rd);
+        cfma4(sY, yd[bb], r);
+        cfma4(sX, y[bb], v);
     }
-    cfma(dY, y0d, Q[0][0] * xa[0] + Q[1][0] * xa[1] + Q[2][0] * xa[2]);
-    cfma(dY, y1d, Q[0][1] * xa[0] + Q[1][1] * xa[1] + Q[2][1] * xa[2]);
+    cfma4(dB, t, sB);
+    cfma4(dY, t, sY);
+    cfma4(dX, t, sX);
 }
 
 // The tangent stream of one scattering matrix (rows il = 1..ilx, t-matrix two rows ahead), complex derivatives.
@@ -624,8 +645,7 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
         // rows past ilx read the table's row ilx (inside the table, finite) and add zero
         const cx<T> tr = r <= ilx ? t : mk<T>(T(0), T(0));
         const size_t rw = (size_t)(r <= ilx ? r : ilx) * FM_NREP;
-        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * tr, ya[1] * tr, yd[0] * tr, yd[1] * tr, Wl + rw, Wdl + rw, dX,
-                                  dY, dB);
+        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya, yd, tr, Wl + rw, Wdl + rw, dX, dY, dB);
         t = tn;
         tn = tnn;
     }
@@ -639,7 +659,7 @@ __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck
         Y.step(r, ya, yd);
         const size_t rn = (size_t)(r + 2 < ilx ? r + 2 : ilx) * tlstride;
         const cx<T> tnn = mk<T>(tl[rn], tl[rn + 1]);
-        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya[0] * t, ya[1] * t, yd[0] * t, yd[1] * t, wv, wdv, dX, dY, dB);
+        bwd_row_c<T, SET, NR, NC>(K, xa, xd, ya, yd, t, wv, wdv, dX, dY, dB);
         t = tn;
         tn = tnn;
     };
@@ -692,7 +712,7 @@ __device__ __forceinline__ void prefix(const PathLDS &g, const cx<T> (&F)[NL - 1
             for (int i = 0; i < L; ++i) {
                 cx<T> acc = mk<T>(T(0), T(0));
 #pragma unroll
-                for (int j = 0; j < L; ++j) cfma(acc, fent<T, SET>(F[t - 1], ph, i, j), col[j]);
+                for (int j = 0; j < L; ++j) cfma4(acc, fent<T, SET>(F[t - 1], ph, i, j), col[j]);
                 P[i][b] = acc;
             }
         }
@@ -728,14 +748,14 @@ __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, 
                 cx<T> m = mk<T>(T(0), T(0));
                 if constexpr (S >= 2) {
 #pragma unroll
-                    for (int a = 0; a < L0; ++a) cfma(m, P[l2][a], U[a][l1]);
+                    for (int a = 0; a < L0; ++a) cfma4(m, P[l2][a], U[a][l1]);
                 } else {
                     m = U[l2][l1];
                 }
                 const cx<T> h = m * ph[l1];
                 if (PD.sg[l1][l2] > 0) K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] + h;
                 else K[PD.idx[l1][l2]] = K[PD.idx[l1][l2]] - h;
-                if (D.M[l1] != 0) cfma(de, h * T(PD.sg[l1][l2] * D.M[l1]), F[S - 1][PD.idx[l1][l2]]);
+                if (D.M[l1] != 0) cfma4(de, h * T(PD.sg[l1][l2] * D.M[l1]), F[S - 1][PD.idx[l1][l2]]);
             }
         }
         ds.put(GK_H, S + 1, mk<T>(de.i, -de.r));
@@ -752,7 +772,7 @@ __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, 
 #pragma unroll
                         for (int i = 0; i < NR; ++i) {
                             const cx<T> f = F[S - 1][PD.idx[i][j]];
-                            if (PD.sg[i][j] > 0) cfma(v, row[i], f); else cfma(v, row[i], mk<T>(-f.r, -f.i));
+                            if (PD.sg[i][j] > 0) cfma4(v, row[i], f); else cfma4(v, row[i], mk<T>(-f.r, -f.i));
                         }
                     }
                     U[a][j] = v;
@@ -835,7 +855,7 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
             for (int b = 0; b < L0; ++b) {
                 cx<T> acc = mk<T>(T(0), T(0));
 #pragma unroll
-                for (int j = 0; j < L; ++j) cfma(acc, fent<T, SET>(F[NL - 2], ph, i, j), P[j][b]);
+                for (int j = 0; j < L; ++j) cfma4(acc, fent<T, SET>(F[NL - 2], ph, i, j), P[j][b]);
                 Pn[i][b] = acc;
             }
     }
@@ -882,8 +902,8 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
             if (c >= 0) {
                 const cx<T> w = mk<T>(chw[((size_t)c * E + ee) * 2], chw[((size_t)c * E + ee) * 2 + 1]) * T(2 * r - 1);
                 accum_row<T, SET, L0, L0>(Tt, xa, ya[0] * w, ya[1] * w, Wn + (size_t)r * FM_NREP);
-                bwd_row_c<T, SET, L0, L0>(K, xa, xd, ya[0] * w, ya[1] * w, yd[0] * w, yd[1] * w,
-                                          Wn + (size_t)r * FM_NREP, Wdn + (size_t)r * FM_NREP, dX, dY, dB);
+                bwd_row_c<T, SET, L0, L0>(K, xa, xd, ya, yd, w, Wn + (size_t)r * FM_NREP, Wdn + (size_t)r * FM_NREP,
+                                          dX, dY, dB);
             }
         }
         ds.put(GK_X, 1, dX);
