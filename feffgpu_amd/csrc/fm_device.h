@@ -427,9 +427,11 @@ __device__ __forceinline__ LegGeoD leg_geometry_dual(const Dd (&pm)[3], const Dd
 // the recursions of Hank3/Hank2 differentiated row by row.
 template <class T> struct Hank3T {
     cx<T> z, zd, a2, a1, b2, b1, c2, c1, a2d, a1d, b2d, b1d, c2d, c1d;
+    T mr;   // -1 / r
     __device__ __forceinline__ void init(cx<T> z_, T r) {
         z = z_;
-        zd = z * (T(-1) / r);
+        mr = T(-1) / r;
+        zd = z * mr;
         a2 = a1 = b2 = b1 = c2 = c1 = a2d = a1d = b2d = b1d = c2d = c1d = mk<T>(T(0), T(0));
     }
     __device__ __forceinline__ void shift(const cx<T> (&o)[3], const cx<T> (&od)[3]) {
@@ -463,25 +465,27 @@ template <class T> struct Hank3T {
         }
         shift(o, od);
     }
+    // with zt = (2r - 3) z and d zt / dr = -zt / r: od = a2d - zt (s' - s / r)
     __device__ __forceinline__ void step(int r, cx<T> (&o)[3], cx<T> (&od)[3]) {
-        const T k = T(2 * r - 3);
-        const cx<T> zt = z * k, ztd = zd * k;
+        const cx<T> zt = z * T(2 * r - 3);
         const cx<T> s0 = a1, s1 = b1 + a1, s2 = c1 + b1;
         const cx<T> s0d = a1d, s1d = b1d + a1d, s2d = c1d + b1d;
         o[0] = cfms(a2, zt, s0);
         o[1] = cfms(b2, zt, s1);
         o[2] = cfms(c2, zt, s2);
-        od[0] = cfms(cfms(a2d, ztd, s0), zt, s0d);
-        od[1] = cfms(cfms(b2d, ztd, s1), zt, s1d);
-        od[2] = cfms(cfms(c2d, ztd, s2), zt, s2d);
+        od[0] = cfms(a2d, zt, mk<T>(s0d.r + s0.r * mr, s0d.i + s0.i * mr));
+        od[1] = cfms(b2d, zt, mk<T>(s1d.r + s1.r * mr, s1d.i + s1.i * mr));
+        od[2] = cfms(c2d, zt, mk<T>(s2d.r + s2.r * mr, s2d.i + s2.i * mr));
         shift(o, od);
     }
 };
 template <class T> struct Hank2T {
     cx<T> z, zd, a2, a1, b2, b1, a2d, a1d, b2d, b1d;
+    T mr;   // -1 / r
     __device__ __forceinline__ void init(cx<T> z_, T r) {
         z = z_;
-        zd = z * (T(-1) / r);
+        mr = T(-1) / r;
+        zd = z * mr;
         a2 = a1 = b2 = b1 = a2d = a1d = b2d = b1d = mk<T>(T(0), T(0));
     }
     __device__ __forceinline__ void shift(const cx<T> (&o)[2], const cx<T> (&od)[2]) {
@@ -505,13 +509,12 @@ template <class T> struct Hank2T {
         shift(o, od);
     }
     __device__ __forceinline__ void step(int r, cx<T> (&o)[2], cx<T> (&od)[2], bool alive = true) {
-        const T k = T(2 * r - 3);
-        const cx<T> zt = z * k, ztd = zd * k;
+        const cx<T> zt = z * T(2 * r - 3);
         const cx<T> s0 = a1, s1 = b1 + a1, s0d = a1d, s1d = b1d + a1d;
         o[0] = cfms(a2, zt, s0);
         o[1] = cfms(b2, zt, s1);
-        od[0] = cfms(cfms(a2d, ztd, s0), zt, s0d);
-        od[1] = cfms(cfms(b2d, ztd, s1), zt, s1d);
+        od[0] = cfms(a2d, zt, mk<T>(s0d.r + s0.r * mr, s0d.i + s0.i * mr));
+        od[1] = cfms(b2d, zt, mk<T>(s1d.r + s1.r * mr, s1d.i + s1.i * mr));
         if (!alive) { o[0] = o[1] = od[0] = od[1] = mk<T>(T(0), T(0)); }
         shift(o, od);
     }
