@@ -827,21 +827,15 @@ __device__ __forceinline__ void keep_loads(const T *kp, int E, cx<T> (&F)[NL - 1
     if constexpr (S < NL - 1) keep_loads<T, SET, NL, S + 1>(kp, E, F);
 }
 
-// The fused sweep at energy ee: returns cchi and stores the complex derivatives of ptrac in ds. kp: the lane's
-// pointer into the scattering matrices the forward kept (read instead of rebuilt), or NULL.
+// The fused sweep at energy ee from the scattering matrices F of every scatterer (kept by the forward or rebuilt):
+// returns cchi and stores the complex derivatives of ptrac in ds.
 template <class T, int SET, int NL>
 __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab,
-                                              int ee, DSlots<T> ds, const T *kp) {
+                                              int ee, DSlots<T> ds, const cx<T> (&F)[NL - 1][FM_NREP]) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
     const int E = ed.E;
-    cx<T> F[NL - 1][FM_NREP];
-    if (kp) {
-        keep_loads<T, SET, NL, 1>(kp, E, F);
-    } else {
-        fused_fblocks<T, SET, NL, 1>(ed, g, W, ltab, ee, F);
-    }
     // Pre_n (L0 x L0)
     cx<T> Pn[L0][L0];
     {
@@ -1378,22 +1372,34 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(WPE
         // ---- 2. the fused sweep over this wavefront's bands
         const int sj = blk * G + jl;
         const bool vj = sj < a.npath_bucket;
-        for (int b = 0; b < nband; ++b) {
-            if (a.bp.wave[b] != wid) continue;
+        // this lane's energy in band b, and its path's (a duplicate in the launch for lanes past its end) kept rows
+        auto energy_of = [&](int b) {
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
-            const bool on = vj && il < nb;
-            const int e = e0 + (il < nb ? il : nb - 1);
+            return e0 + (il < nb ? il : nb - 1);
+        };
+        auto kept_at = [&](int e) {
+            return reinterpret_cast<const T *>(a.keep) + ((size_t)(vj ? sj : blk * G) * keep_rows<SET>(NL) * E + e) * 2;
+        };
+        auto next_band = [&](int b) {
+            for (++b; b < nband && a.bp.wave[b] != wid; ++b) {}
+            return b;
+        };
+        for (int b = next_band(-1); b < nband; b = next_band(b)) {
+            const int e = energy_of(b);
+            const bool on = vj && il < a.bp.start[b + 1] - a.bp.start[b];
             const size_t reg = on ? (size_t)sj : (size_t)a.npath_bucket;
             const DSlots<T> ds{drs, (uint32_t)((reg * a.dstride + E + e) * sizeof(cx<T>)), sbytes, NL};
             cx<T> c;
             if constexpr (SET >= FM_SET_SS) {
                 c = ss_fused_energy<T, SET - FM_SET_SS + 1>(ed, gj, e, ds);
             } else {
-                // this lane's path (a duplicate in the launch for lanes past its end) and energy in the kept rows
-                const T *kp = a.keep ? reinterpret_cast<const T *>(a.keep) +
-                                           ((size_t)(vj ? sj : blk * G) * keep_rows<SET>(NL) * E + e) * 2
-                                     : nullptr;
-                c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds, kp);
+                cx<T> F[NL - 1][FM_NREP];
+                if (a.keep) {
+                    keep_loads<T, SET, NL, 1>(kept_at(e), E, F);
+                } else {
+                    fused_fblocks<T, SET, NL, 1>(ed, gj, Wj, ltab, e, F);
+                }
+                c = fused_energy<T, SET, NL>(ed, gj, Wj, Wdj, ltab, e, ds, F);
             }
             if (on) wcc[(size_t)sj * a.dstride + e] = c;
         }
