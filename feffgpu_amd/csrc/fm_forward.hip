@@ -114,9 +114,12 @@ __global__ __launch_bounds__(FM_WAVE * 4) __attribute__((amdgpu_waves_per_eu(FM_
         }
         if constexpr (SET < FM_SET_SS) build_tables<T, SET, false, 1>(ed, smem, a.path_bytes, off_W, 0, wid, 4, G, nl, ltab);
         __syncthreads();
-        // ---- energies: the bands of this wavefront
-        for (int b = 0; b < a.bp.nband; ++b) {
-            if (a.bp.wave[b] != wid) continue;
+        // ---- energies: the bands of this wavefront, each wavefront stepping from one of its bands to the next
+        auto next_band = [&](int b) {
+            for (++b; b < a.bp.nband && a.bp.wave[b] != wid; ++b) {}
+            return b;
+        };
+        for (int b = next_band(-1); b < a.bp.nband; b = next_band(b)) {
             const int e0 = a.bp.start[b], nb = a.bp.start[b + 1] - e0;
             const bool active = il < nb && pval[jl];
             const int e = e0 + (il < nb ? il : nb - 1);
