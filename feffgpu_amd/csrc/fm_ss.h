@@ -96,6 +96,15 @@ template <class T> __device__ __forceinline__ dcx<T> operator*(dcx<T> a, dcx<T> 
 template <class T> __device__ __forceinline__ dcx<T> operator*(dcx<T> a, T s) { return {a.v * s, a.d * s}; }
 template <class T> __device__ __forceinline__ dcx<T> operator*(dcx<T> a, cx<T> s) { return {a.v * s, a.d * s}; }
 
+// acc += a * b: complex values as the product, then the sum (the forward); dual values, the single-scattering
+// adjoint's, as four fused multiply-adds per complex product (cfma4)
+template <class T> __device__ __forceinline__ void vfma(cx<T> &acc, cx<T> a, cx<T> b) { acc = acc + a * b; }
+template <class T> __device__ __forceinline__ void vfma(dcx<T> &acc, dcx<T> a, dcx<T> b) {
+    cfma4(acc.v, a.v, b.v);
+    cfma4(acc.d, a.d, b.v);
+    cfma4(acc.d, a.v, b.d);
+}
+
 template <class V> struct VOps;
 template <class T> struct VOps<cx<T>> {
     using R = T;
@@ -145,12 +154,13 @@ template <class V, int C> struct HankRows {
 #pragma unroll
         for (int c = 0; c < C; ++c) { p2[c] = p1[c]; p1[c] = o[c]; }
     }
-    // row r > C + 1 (past every column's explicit rows): the recursion alone, the same arithmetic as row()
-    __device__ __forceinline__ void row_rec(int r, V (&o)[C], bool alive) {
+    // row r > C + 1 (past every column's explicit rows) of a live row: the recursion alone, the same arithmetic as
+    // row()
+    __device__ __forceinline__ void row_rec(int r, V (&o)[C]) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            if (c == 0) o[c] = (p2[0] - (z * p1[0]) * T(2 * r - 3)) * T(alive ? 1 : 0);
-            else o[c] = (p2[c] - (z * (p1[c] + p1[c - 1])) * T(2 * r - 3)) * T(alive ? 1 : 0);
+            if (c == 0) o[c] = p2[0] - (z * p1[0]) * T(2 * r - 3);
+            else o[c] = p2[c] - (z * (p1[c] + p1[c - 1])) * T(2 * r - 3);
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) { p2[c] = p1[c]; p1[c] = o[c]; }
@@ -178,18 +188,20 @@ __device__ __forceinline__ void ss_reps(const fm_edge &ed, int ee, int ip_s, V z
     // compile time; the rest (r <= ilx) by the recursion alone, two rows per iteration
     constexpr int RP = C + 1 > 4 ? C + 1 : 4;
     auto scatter_row = [&](int r, const V (&h)[C]) {
-        const T sgn = (r & 1) ? T(1) : T(-1);          // (-1)^(il - 1)
-        const cx<T> t = mk<T>(tl[(size_t)r * 2 * E], tl[(size_t)r * 2 * E + 1]);
+        const T sgn = (r & 1) ? T(1) : T(-1);          // (-1)^(il - 1), exact on the t-matrix factor
+        const cx<T> t = mk<T>(tl[(size_t)r * 2 * E] * sgn, tl[(size_t)r * 2 * E + 1] * sgn);
         V yt[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) yt[c] = (h[c] * t) * sgn;
+        for (int c = 0; c < 3; ++c) yt[c] = h[c] * t;
 #pragma unroll
         for (int k = 0; k < D.nrep; ++k)
-            if (r >= D.ilmin[k]) Fk[k] = Fk[k] + h[D.ca[k] - 1] * yt[D.cb[k] - 1];
+            if (r >= D.ilmin[k]) vfma(Fk[k], h[D.ca[k] - 1], yt[D.cb[k] - 1]);
     };
+    // rows past RP up to ilx: ilx <= lxa (lxa is the larger lmax + 1 of the two atoms, one of them the
+    // scatterer), so every such row is alive
     auto rec_row = [&](int r) {
         V h[C];
-        H.row_rec(r, h, r <= lxa);
+        H.row_rec(r, h);
         scatter_row(r, h);
     };
 #pragma unroll
