@@ -1519,14 +1519,29 @@ __global__ __launch_bounds__(FM_WAVE * 4) void k_adjoint_tail(BArgs a) {
     }
 }
 
-// grad_pos[atom] = sum of gcoord over the atom's references, in list order
-__global__ void k_atom_grad(int natom, const int32_t *ptr, const int32_t *refs, const double *gcoord, double *grad) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * natom) return;
-    const int atom = t / 3, c = t % 3;
-    double s = 0.0;
-    for (int k = ptr[atom]; k < ptr[atom + 1]; ++k) s += gcoord[(size_t)refs[k] * 3 + c];
-    grad[t] = s;
+// grad_pos[atom] = sum of gcoord over the atom's references: one wavefront per atom (an absorber can appear in
+// thousands of paths), lane l summing references l, l + 64, ... in list order, the 64 partial sums of each
+// component then added in a fixed butterfly. Four atoms per 256-thread block; a wavefront past natom returns whole.
+__global__ __launch_bounds__(FM_WAVE * 4) void k_atom_grad(int natom, const int32_t *ptr, const int32_t *refs,
+                                                           const double *gcoord, double *grad) {
+    const int atom = blockIdx.x * 4 + (int)(threadIdx.x / FM_WAVE);
+    if (atom >= natom) return;
+    const int lane = lane_id();
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = ptr[atom] + lane; k < ptr[atom + 1]; k += FM_WAVE) {
+        const double *g = gcoord + (size_t)refs[k] * 3;
+        s0 += g[0];
+        s1 += g[1];
+        s2 += g[2];
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        grad[(size_t)atom * 3] = s0;
+        grad[(size_t)atom * 3 + 1] = s1;
+        grad[(size_t)atom * 3 + 2] = s2;
+    }
 }
 
 // GPUPathStage.chi_fine's adjoint on its own: one wave per path.
@@ -1950,8 +1965,8 @@ int fm_chi_paths_backward(const fm_edge *edges, int32_t nedge, const fm_paths *p
     }
     if (grad_pos && natom > 0) {
         if (!atom_ptr || !atom_refs) return set_err(FM_EINVAL, "NULL atom lists");
-        const int nt = 3 * natom;
-        hipLaunchKernelGGL(k_atom_grad, dim3((nt + 255) / 256), dim3(256), 0, st, natom, atom_ptr, atom_refs, gcoord, grad_pos);
+        hipLaunchKernelGGL(k_atom_grad, dim3((natom + 3) / 4), dim3(FM_WAVE * 4), 0, st, natom, atom_ptr, atom_refs,
+                           gcoord, grad_pos);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_err(e, "atom gradient launch");
     }
