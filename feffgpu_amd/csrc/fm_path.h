@@ -804,12 +804,14 @@ __global__ __launch_bounds__(FM_WAVE *SEG_WAVES) void k_segment_sum(int nseg, in
                 const int t = tb + u * SEG_WAVES;
                 pv[u] = t < t1 ? seg_paths[t] : -1;
             }
+            // unconditional loads (past the list's end they read path 0, a valid row, and are not summed), so that
+            // the compiler issues all U of them before the first wait instead of branching around each
             double dv[U], cv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int p = pv[u] < 0 ? 0 : pv[u];
-                dv[u] = pv[u] < 0 ? 0.0 : deg[p];
-                cv[u] = pv[u] < 0 ? 0.0 : (double)chi_path[(size_t)p * nk + k];
+                dv[u] = deg[p];
+                cv[u] = (double)chi_path[(size_t)p * nk + k];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
