@@ -272,19 +272,23 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
 template <int B> struct RotCoef {
     typedef double vec __attribute__((ext_vector_type(B)));   // vector values, not arrays: kept in registers
     vec c0, c1, c2, nr;
+    // The loads are unconditional (a row past `rows`, or below l = 3, reads a row inside the tables, rotn row
+    // min(il, rows) <= L1 and rotc row min(max(l - 1, 2), L1)) and the values are zeroed after, so that the
+    // compiler issues all of them before the first wait instead of branching around each.
     __device__ __forceinline__ void fetch(const fm_edge &ed, int il0, int rows, int mm, int nn) {
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const int il = il0 + u, l = il - 1;
-            nr[u] = il <= rows ? ed.rotn[(size_t)il * 16 + nn] : 0.0;
-            if (il <= rows && l >= 3) {
-                const double *co = ed.rotc + ((size_t)(l - 1) * 49 + mm) * 3;
-                c0[u] = co[0];
-                c1[u] = co[1];
-                c2[u] = co[2];
-            } else {
-                c0[u] = c1[u] = c2[u] = 0.0;
-            }
+            const bool live = il <= rows, rec = live && l >= 3;
+            const int ilc = live ? il : rows;
+            const int lrow = min(ilc - 1 >= 3 ? ilc - 2 : 2, ed.L1);
+            const double n0 = ed.rotn[(size_t)ilc * 16 + nn];
+            const double *co = ed.rotc + ((size_t)lrow * 49 + mm) * 3;
+            const double a0 = co[0], a1 = co[1], a2 = co[2];
+            nr[u] = live ? n0 : 0.0;
+            c0[u] = rec ? a0 : 0.0;
+            c1[u] = rec ? a1 : 0.0;
+            c2[u] = rec ? a2 : 0.0;
         }
     }
 };
