@@ -623,17 +623,18 @@ __device__ __forceinline__ void bwd_row_c(const cx<T> (&K)[FM_NREP], const cx<T>
 template <class T, int SET, int NR, int NC>
 __device__ __forceinline__ void fblock_bwd_c(const cx<T> (&K)[FM_NREP], cx<T> ck, T rX, T rY, int ilx,
                                              const T *__restrict__ Wl, const T *__restrict__ Wdl,
-                                             const T *__restrict__ tl, int tlstride, cx<T> &dX, cx<T> &dY, cx<T> &dB) {
+                                             const T *__restrict__ tl, int tlstride, cx<T> t1, cx<T> t2, cx<T> &dX,
+                                             cx<T> &dY, cx<T> &dB) {
     dX = dY = dB = mk<T>(T(0), T(0));
     Hank3T<T> X;
     Hank2T<T> Y;
     X.init(minus_i_over(ck * rX), rX);
     Y.init(minus_i_over(ck * rY), rY);
     cx<T> xa[3], xd[3], ya[2], yd[2];
-    // the t-matrix two rows ahead (rows past ilx repeat row ilx, which stays in the table)
-    cx<T> t = mk<T>(tl[tlstride], tl[tlstride + 1]);
-    const size_t r2 = (size_t)(ilx < 2 ? ilx : 2) * tlstride;
-    cx<T> tn = mk<T>(tl[r2], tl[r2 + 1]);
+    // the t-matrix two rows ahead (rows past ilx repeat row ilx, which stays in the table); rows 1 and min(2, ilx)
+    // come from the caller
+    cx<T> t = t1;
+    cx<T> tn = t2;
     // rows 1..4 unrolled (their seeds are explicit per row); a lane whose energy has fewer rows runs them with the
     // t-matrix factor zero, which adds nothing
 #pragma unroll
@@ -719,18 +720,36 @@ __device__ __forceinline__ void prefix(const PathLDS &g, const cx<T> (&F)[NL - 1
     }
 }
 
+// What a chain step needs of its scatterer S at the lane's energy: the partial waves ilx, the scatterer's t-matrix
+// column and its rows 1 and min(2, ilx) (row 2 read whole from min(2, L1), used only when ilx >= 2).
+template <class T> struct ScatIn {
+    int ilx;
+    const T *tl;
+    cx<T> t1, t2;
+};
+template <class T>
+__device__ __forceinline__ ScatIn<T> scat_in(const fm_edge &ed, const PathLDS &g, int ltab, int ee, int S) {
+    const int E = ed.E, ip = g.ipot[S];
+    const T *tl = TL<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
+    const size_t r2 = (size_t)(ed.L1 < 2 ? ed.L1 : 2) * (2 * E);
+    return ScatIn<T>{ed.lmax[(size_t)ip * E + ee] + 1, tl, mk<T>(tl[2 * E], tl[2 * E + 1]), mk<T>(tl[r2], tl[r2 + 1])};
+}
+// Whether the sweep reads every scatterer's ScatIn when the energy starts (their loads in flight together, hidden by
+// Pre_n) or at its chain step: early where the registers hold them without more spills (measured: K edges up to 4
+// legs, L/M edges 3 legs).
+template <int SET, int NL> constexpr bool scat_early() { return NL <= (SET == 0 ? 4 : 3); }
+
 // One chain step of the fused sweep (scatterer S; NR x NC block): K and the eta derivative from Pre_S and U, then
 // U <- U F_S (unless S = 1), then the tangent stream of F_S.
 template <class T, int SET, int NL, int S>
 __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee,
-                                           const cx<T> (&F)[NL - 1][FM_NREP],
+                                           cx<T> ck, const ScatIn<T> (&sc)[NL - 1], const cx<T> (&F)[NL - 1][FM_NREP],
                                            cx<T> (&U)[lamdef(SET).L0][lamdef(SET).L], DSlots<T> ds) {
     constexpr LamDef D = lamdef(SET);
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
     constexpr int NR = S == NL - 1 ? L0 : L, NC = S == 1 ? L0 : L;
     const int E = ed.E;
-    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
     cx<T> ph[6];
     row_phases<T, SET>(ph, g, S + 1);
     cx<T> K[FM_NREP];
@@ -780,12 +799,13 @@ __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, 
             }
         }
     }
-    const int ip = g.ipot[S];
-    const int ilx = ed.lmax[(size_t)ip * E + ee] + 1;
-    const T *tl = TL<T>::tl(ed) + ((size_t)ip * ltab) * (2 * E) + 2 * ee;
+    ScatIn<T> si;
+    if constexpr (scat_early<SET, NL>()) si = sc[S - 1];
+    else si = scat_in<T>(ed, g, ltab, ee, S);
     cx<T> dX, dY, dB;
-    fblock_bwd_c<T, SET, NR, NC>(K, ck, (T)g.r[S + 1], (T)g.r[S], ilx, W + (size_t)(S - 1) * ltab * FM_NREP,
-                                 Wd + (size_t)(S - 1) * ltab * FM_NREP, tl, 2 * E, dX, dY, dB);
+    fblock_bwd_c<T, SET, NR, NC>(K, ck, (T)g.r[S + 1], (T)g.r[S], si.ilx, W + (size_t)(S - 1) * ltab * FM_NREP,
+                                 Wd + (size_t)(S - 1) * ltab * FM_NREP, si.tl, 2 * E, si.t1,
+                                 si.ilx >= 2 ? si.t2 : si.t1, dX, dY, dB);
     ds.put(GK_X, S + 1, dX);
     ds.put(GK_Y, S, dY);
     ds.put(GK_B, S, dB);
@@ -793,10 +813,10 @@ __device__ __forceinline__ void fused_step(const fm_edge &ed, const PathLDS &g, 
 
 template <class T, int SET, int NL, int S>
 __device__ __forceinline__ void fused_chain(const fm_edge &ed, const PathLDS &g, const T *W, const T *Wd, int ltab, int ee,
-                                            const cx<T> (&F)[NL - 1][FM_NREP],
+                                            cx<T> ck, const ScatIn<T> (&sc)[NL - 1], const cx<T> (&F)[NL - 1][FM_NREP],
                                             cx<T> (&U)[lamdef(SET).L0][lamdef(SET).L], DSlots<T> ds) {
-    fused_step<T, SET, NL, S>(ed, g, W, Wd, ltab, ee, F, U, ds);
-    if constexpr (S > 1) fused_chain<T, SET, NL, S - 1>(ed, g, W, Wd, ltab, ee, F, U, ds);
+    fused_step<T, SET, NL, S>(ed, g, W, Wd, ltab, ee, ck, sc, F, U, ds);
+    if constexpr (S > 1) fused_chain<T, SET, NL, S - 1>(ed, g, W, Wd, ltab, ee, ck, sc, F, U, ds);
 }
 
 template <class T, int SET, int NL, int S>
@@ -836,6 +856,26 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
     constexpr PairDef PD = pairdef(SET);
     constexpr int L = D.L, L0 = D.L0;
     const int E = ed.E;
+    // read ahead of Pre_n, whose work hides their latency: the termination's channel weights of rows 1..4 (a row
+    // without a channel reads channel 0's and does not use it), ck, and (scat_early) every scatterer's ScatIn
+    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
+    ScatIn<T> sc[NL - 1];
+    if constexpr (scat_early<SET, NL>()) {
+#pragma unroll
+        for (int S = 1; S < NL; ++S) sc[S - 1] = scat_in<T>(ed, g, ltab, ee, S);
+    }
+    const T *chw = TL<T>::chw(ed);
+    int chan[4];
+    cx<T> wraw[4];
+#pragma unroll
+    for (int r = 1; r <= 4; ++r) {
+        int c = -1;
+        for (int q = 0; q < ed.nchan; ++q)
+            if (ed.chan_il[q] == r) c = q;
+        chan[r - 1] = c;
+        const size_t o = ((size_t)(c >= 0 ? c : 0) * E + ee) * 2;
+        wraw[r - 1] = mk<T>(chw[o], chw[o + 1]);
+    }
     // Pre_n (L0 x L0)
     cx<T> Pn[L0][L0];
     {
@@ -854,7 +894,6 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
             }
     }
     // termination: T with its tangents against K_T[a][b] = Pn[b][a] ph1[a]
-    const cx<T> ck = mk<T>((T)ed.ck[2 * ee], (T)ed.ck[2 * ee + 1]);
     cx<T> U[L0][L];
     cx<T> ptrac = mk<T>(T(0), T(0));
     {
@@ -883,18 +922,14 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
         for (int k = 0; k < FM_NREP; ++k) Tt[k] = mk<T>(T(0), T(0));
         const T *Wn = W + (size_t)(NL - 1) * ltab * FM_NREP;
         const T *Wdn = Wd + (size_t)(NL - 1) * ltab * FM_NREP;
-        const T *chw = TL<T>::chw(ed);
         cx<T> dX = mk<T>(T(0), T(0)), dY = dX, dB = dX;
         cx<T> xa[3], xd[3], ya[2], yd[2];
 #pragma unroll
         for (int r = 1; r <= 4; ++r) {
             X.seed_row(r, xa, xd, r <= lx1);
             if (r < 4) Y.seed_row(r, ya, yd, r <= lxn); else Y.step(r, ya, yd, r <= lxn);
-            int c = -1;
-            for (int q = 0; q < ed.nchan; ++q)
-                if (ed.chan_il[q] == r) c = q;
-            if (c >= 0) {
-                const cx<T> w = mk<T>(chw[((size_t)c * E + ee) * 2], chw[((size_t)c * E + ee) * 2 + 1]) * T(2 * r - 1);
+            if (chan[r - 1] >= 0) {
+                const cx<T> w = wraw[r - 1] * T(2 * r - 1);
                 accum_row<T, SET, L0, L0>(Tt, xa, ya[0] * w, ya[1] * w, Wn + (size_t)r * FM_NREP);
                 bwd_row_c<T, SET, L0, L0>(K, xa, xd, ya, yd, w, Wn + (size_t)r * FM_NREP, Wdn + (size_t)r * FM_NREP,
                                           dX, dY, dB);
@@ -918,7 +953,7 @@ __device__ __forceinline__ cx<T> fused_energy(const fm_edge &ed, const PathLDS &
             }
         ds.put(GK_H, 1, mk<T>(de.i, -de.r));
     }
-    fused_chain<T, SET, NL, NL - 1>(ed, g, W, Wd, ltab, ee, F, U, ds);
+    fused_chain<T, SET, NL, NL - 1>(ed, g, W, Wd, ltab, ee, ck, sc, F, U, ds);
     return ptrac * cfac_at<T>(ed, g, ee, NL);
 }
 
