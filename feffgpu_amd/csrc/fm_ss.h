@@ -184,15 +184,34 @@ __device__ __forceinline__ void ss_reps(const fm_edge &ed, int ee, int ip_s, V z
     HankRows<V, C> H;
     H.init(z);
     const T *tl = tlbase + ((size_t)ip_s * ltab) * (2 * E) + 2 * ee;
+    // the t-matrix two rows ahead (rows past ilx read row ilx, inside the table, and are not used), and the
+    // termination's channel weights of rows 1..4 read with the first rows (a row without a channel reads channel 0's
+    // and does not use it)
+    auto trow = [&](int r) {
+        const size_t o = (size_t)(r < ilx ? r : ilx) * 2 * E;
+        return mk<T>(tl[o], tl[o + 1]);
+    };
+    int chan[4];
+    cx<T> wraw[4];
+#pragma unroll
+    for (int r = 1; r <= 4; ++r) {
+        int c = -1;
+        for (int q = 0; q < ed.nchan; ++q)
+            if (ed.chan_il[q] == r) c = q;
+        chan[r - 1] = c;
+        const size_t o = ((size_t)(c >= 0 ? c : 0) * E + ee) * 2;
+        wraw[r - 1] = mk<T>(chw[o], chw[o + 1]);
+    }
+    cx<T> t = trow(1), tn = trow(2);
     // rows 1 .. RP (the columns' explicit rows and the termination's) unrolled, so that their cases resolve at
     // compile time; the rest (r <= ilx) by the recursion alone, two rows per iteration
     constexpr int RP = C + 1 > 4 ? C + 1 : 4;
-    auto scatter_row = [&](int r, const V (&h)[C]) {
+    auto scatter_row = [&](int r, const V (&h)[C], cx<T> traw) {
         const T sgn = (r & 1) ? T(1) : T(-1);          // (-1)^(il - 1), exact on the t-matrix factor
-        const cx<T> t = mk<T>(tl[(size_t)r * 2 * E] * sgn, tl[(size_t)r * 2 * E + 1] * sgn);
+        const cx<T> tr = mk<T>(traw.r * sgn, traw.i * sgn);
         V yt[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) yt[c] = h[c] * t;
+        for (int c = 0; c < 3; ++c) yt[c] = h[c] * tr;
 #pragma unroll
         for (int k = 0; k < D.nrep; ++k)
             if (r >= D.ilmin[k]) vfma(Fk[k], h[D.ca[k] - 1], yt[D.cb[k] - 1]);
@@ -202,20 +221,23 @@ __device__ __forceinline__ void ss_reps(const fm_edge &ed, int ee, int ip_s, V z
     auto rec_row = [&](int r) {
         V h[C];
         H.row_rec(r, h);
-        scatter_row(r, h);
+        const cx<T> tnn = trow(r + 2);
+        scatter_row(r, h, t);
+        t = tn;
+        tn = tnn;
     };
 #pragma unroll
     for (int r = 1; r <= RP; ++r) {
         V h[C];
         H.row(r, h, r <= lxa);
         const T sgn = (r & 1) ? T(1) : T(-1);          // (-1)^(il - 1)
-        if (r <= ilx) scatter_row(r, h);
+        const cx<T> tnn = trow(r + 2);
+        if (r <= ilx) scatter_row(r, h, t);
+        t = tn;
+        tn = tnn;
         if (r <= 4) {
-            int ch = -1;
-            for (int q = 0; q < ed.nchan; ++q)
-                if (ed.chan_il[q] == r) ch = q;
-            if (ch >= 0) {
-                const cx<T> w = mk<T>(chw[((size_t)ch * E + ee) * 2], chw[((size_t)ch * E + ee) * 2 + 1]) * (T(2 * r - 1) * sgn);
+            if (chan[r - 1] >= 0) {
+                const cx<T> w = wraw[r - 1] * (T(2 * r - 1) * sgn);
                 V yw[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) yw[c] = h[c] * w;
