@@ -272,10 +272,24 @@ __device__ inline int load_path(const PathSrc &a, int p, PathLDS &g, int &edge_i
 template <int B> struct RotCoef {
     typedef double vec __attribute__((ext_vector_type(B)));   // vector values, not arrays: kept in registers
     vec c0, c1, c2, nr;
-    // The loads are unconditional (a row past `rows`, or below l = 3, reads a row inside the tables, rotn row
-    // min(il, rows) <= L1 and rotc row min(max(l - 1, 2), L1)) and the values are zeroed after, so that the
-    // compiler issues all of them before the first wait instead of branching around each.
+    // B > 1: the loads are unconditional (a row past `rows`, or below l = 3, reads a row inside the tables, rotn
+    // row min(il, rows) <= L1 and rotc row min(max(l - 1, 2), L1)) and the values are zeroed after, so that the
+    // compiler issues all of them before the first wait instead of branching around each. B = 1 (the forward's
+    // tables, one row at a time) loads only what it uses (measured faster there).
     __device__ __forceinline__ void fetch(const fm_edge &ed, int il0, int rows, int mm, int nn) {
+        if constexpr (B == 1) {
+            const int il = il0, l = il - 1;
+            nr[0] = il <= rows ? ed.rotn[(size_t)il * 16 + nn] : 0.0;
+            if (il <= rows && l >= 3) {
+                const double *co = ed.rotc + ((size_t)(l - 1) * 49 + mm) * 3;
+                c0[0] = co[0];
+                c1[0] = co[1];
+                c2[0] = co[2];
+            } else {
+                c0[0] = c1[0] = c2[0] = 0.0;
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const int il = il0 + u, l = il - 1;
